@@ -1,0 +1,222 @@
+"""Benchmark: voxel-iterations/s of one VPFFT strain step on one MI355X.
+
+Workload (the headline configuration): 256^3 periodic-Voronoi polycrystal,
+4096 grains with Haar-random orientations, FCC, n=20, tau0=1, gdot0=1, a seeded
+random symmetric traceless strain rate of Frobenius norm 1, tolerance 1e-6,
+at most 100 fixed-point iterations, dt=0.005; consecutive strain steps (the
+reference's RunVPFFT loop).  value = N * iterations / wall time over the timed
+steps (plain FP64 form).  The mixed form (FP32 FFT + Green, FP64 Newton) runs
+the same steps from the same initial state and reports its speedup and its
+macroscopic-stress error against the FP64 form.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] ...
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# Algorithmic work per unit, matching the kernels' own arithmetic (the
+# derivation is in DESIGN.md, "Roofline accounting"):
+NR_FLOPS_FIXED = 1500.0       # per voxel per iteration, outside the Newton steps
+NR_FLOPS_PER_STEP = 1150.0    # per Newton step
+NR_BYTES = 408.0              # d~, sigma, lambda, R, crss read; sigma, lambda, phi written (FP64)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def fft_bytes(kernel, n, nc, elt):
+    """Algorithmic HBM bytes of one FFT pass launch (elt = bytes per real)."""
+    if kernel in ("fft_z_r2c", "fft_z_c2r"):
+        return 5 * n * elt + 5 * nc * 2 * elt
+    return 2 * 5 * nc * 2 * elt  # y / x passes: read + write the half spectrum
+
+
+def run_form(V, R, mat, edot, args, precision, steps, warmup, profile):
+    n = R.shape[0]
+    g = V.MaterialGrid(args.size, args.size, args.size, mat, precision=precision)
+    g.SendDataAndInitialize(R)
+    for _ in range(warmup):
+        g.RunSingleStrainStep(edot, args.max_iter, args.dt, args.tol)
+    # restart from the initial state so both forms time the same steps
+    g.SendDataAndInitialize(R)
+    g.SetVoxelLength(1.0, 1.0, 1.0)
+    if profile:
+        g.SetProfiling(True)
+    iters, avgs, infos = 0, [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        avgs.append(g.RunSingleStrainStep(edot, args.max_iter, args.dt, args.tol))
+        infos.append(g.last_info.as_dict())
+        iters += g.last_info.iterations
+    wall = time.perf_counter() - t0
+    ktimes = g.KernelTimes() if profile else {}
+    g.close()
+    return dict(wall=wall, iters=iters, vox_iters=n * iters, avgs=avgs, infos=infos, ktimes=ktimes)
+
+
+def cpu_baseline(V, args, edot):
+    """The oracle's fixed-point loop on a bounded sample of the workload."""
+    import ctypes as C
+
+    from oracle import oracle as O
+    s = args.cpu_size
+    grains = max(8, int(round(args.grains * (s / args.size) ** 3)))
+    R, _, _ = V.generate_voronoi(s, s, s, grains, args.seed)
+    n = R.shape[0]
+    mat = V.Material.uniform(1.0, args.nexp)
+    lib = O.load()
+    m = O.make_material(s, s, s, mat)
+    mall = O.schmid_all(R)
+    crss = np.ones((n, 12))
+    sig, lam = np.zeros((n, 5)), np.zeros((n, 5))
+    phi, dtl = np.zeros((5, n)), np.zeros((5, n))
+    p = O._p
+    lib.vo_initialize(C.byref(m), p(mall), p(crss), p(edot), p(sig), p(lam), p(phi))
+    L0 = np.zeros(25)
+    lib.vo_build_L0(C.byref(m), p(mall), p(crss), p(sig), p(L0))
+    its = 0
+    t0 = time.perf_counter()
+    while its < args.cpu_max_iters and (time.perf_counter() - t0) < args.cpu_seconds:
+        lib.vo_strain_rate_evolution(C.byref(m), p(L0), p(phi), p(dtl))
+        S, mx = np.zeros(12), np.zeros(1)
+        lib.vo_voxel_update(C.byref(m), p(mall), p(crss), p(edot), p(L0), p(dtl), p(sig), p(lam), p(phi), p(S), p(mx))
+        its += 1
+    wall = time.perf_counter() - t0
+    return {
+        "value": n * its / wall, "unit": "voxel-iters/s", "cores": O.num_threads(), "kind": "port",
+        "sample": f"oracle fixed-point loop (FFT + literal Green + per-voxel NR) on a {s}^3 grid, "
+                  f"{grains} grains, n={args.nexp}, same strain rate; {its} iterations in {wall:.1f} s; "
+                  "prologue excluded",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--grains", type=int, default=4096)
+    ap.add_argument("--nexp", type=int, default=20)
+    ap.add_argument("--max-iter", type=int, default=100)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--dt", type=float, default=0.005)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-size", type=int, default=96)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-max-iters", type=int, default=20)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU: the whole voxel grid lives in one MI355X's HBM and there is no "
+              "multi-GPU decomposition in this build (--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import vpfft_amd as V
+    V.load_library()
+
+    log(f"[bench] generating {args.size}^3 microstructure, {args.grains} grains")
+    R, _, _ = V.generate_voronoi(args.size, args.size, args.size, args.grains, args.seed)
+    rng = np.random.default_rng(args.seed)
+    edot = rng.standard_normal(5)
+    edot /= np.linalg.norm(edot)
+    mat = V.Material.uniform(1.0, args.nexp)
+    n = R.shape[0]
+
+    log("[bench] FP64 form")
+    f64 = run_form(V, R, mat, edot, args, V.FP64, args.steps, args.warmup, profile=True)
+    log(f"[bench] FP64: {f64['iters']} iterations in {f64['wall']:.3f} s")
+    value = f64["vox_iters"] / f64["wall"]
+
+    # roofline of the dominant kernel, from the HIP-event times of the timed steps
+    kt = f64["ktimes"]
+    dom = max(kt, key=lambda k: kt[k][0])
+    nc = args.size * args.size * (args.size // 2 + 1)
+    nr_steps_total = sum(i["nr_steps_total"] for i in f64["infos"])
+    per_kernel = {}
+    for k, (ms, cnt) in kt.items():
+        if cnt == 0:
+            continue
+        entry = {"ms_total": ms, "launches": cnt, "ms_avg": ms / cnt}
+        if k.startswith("fft"):
+            b = fft_bytes(k, n, nc, 8)
+            entry["GBps"] = b / (ms / cnt) / 1e6
+        if k == "nr_update":
+            flops = n * f64["iters"] * NR_FLOPS_FIXED + nr_steps_total * NR_FLOPS_PER_STEP
+            entry["TFLOPs"] = flops / (ms / 1e3) / 1e12
+            entry["GBps"] = NR_BYTES * n / (ms / cnt) / 1e6
+        per_kernel[k] = entry
+    if dom == "nr_update":
+        ach = per_kernel[dom]["TFLOPs"]
+        roof = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "nr_update",
+                "note": "FP64 vector ALU (no MFMA): MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s"}
+    else:
+        ach = per_kernel[dom]["GBps"]
+        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom}
+
+    mixed = None
+    if not args.no_mixed:
+        log("[bench] mixed form")
+        mx = run_form(V, R, mat, edot, args, V.MIXED, args.steps, args.warmup, profile=False)
+        v_mx = mx["vox_iters"] / mx["wall"]
+        err = float(np.linalg.norm(mx["avgs"][0] - f64["avgs"][0]) / np.linalg.norm(f64["avgs"][0]))
+        mixed = {"value": v_mx, "speedup_vs_fp64": v_mx / value, "iterations": mx["iters"],
+                 "macro_stress_rel_err_step1": err, "ms_per_step": 1e3 * mx["wall"] / args.steps}
+        log(f"[bench] mixed: {mx['iters']} iterations in {mx['wall']:.3f} s, <sigma> err {err:.2e}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle)")
+        cpu = cpu_baseline(V, args, edot)
+
+    loop_ms = sum(i["ms_loop"] for i in f64["infos"])
+    out = {
+        "metric": "voxel-iters/s at 256^3 (plain FP64 vs mixed @1e-5 stress err); RVE p99 latency",
+        "value": value,
+        "unit": "voxel-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * f64["wall"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"one VPFFT strain step per step: {args.size}^3 periodic Voronoi, {args.grains} grains, "
+                        f"FCC n={args.nexp}, random unit traceless strain rate, tol {args.tol}, "
+                        f"max {args.max_iter} iterations, dt {args.dt}",
+            "grid": args.size, "grains": args.grains, "rate_sensitivity": args.nexp, "tol": args.tol,
+            "max_iter": args.max_iter, "iterations": f64["iters"], "precision": "fp64",
+        },
+        "loop_only_voxel_iters_per_s": f64["vox_iters"] / (loop_ms / 1e3),
+        "roofline": roof,
+        "kernels": per_kernel,
+        "mixed": mixed,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

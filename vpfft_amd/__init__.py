@@ -1,0 +1,447 @@
+"""vpfft_amd -- MI355X VPFFT strain-step engine, Python host mirror.
+
+The compute path is libvpfft.so (hand-written gfx950 HIP kernels behind a C
+ABI, include/vpfft.h).  This module is a thin ctypes binding that mirrors the
+reference's host interface, VPFFT::DataStructures::MaterialGrid
+(Src/MaterialGrid.h:59-577), with the same method names and argument
+meanings, so a driver written against the reference reads the same here.
+
+There is no CPU fallback: if libvpfft.so is missing, or no GPU is present,
+the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvpfft.so")
+
+NSLIP = 12
+FP64, MIXED = 0, 1
+
+FIELD_ORIENTATION = 0
+FIELD_CRSS = 1
+FIELD_STRESS = 2
+FIELD_LAGRANGE = 3
+FIELD_POLARIZATION = 4
+FIELD_STRAIN_RATE_VARIATION = 5
+FIELD_ACC_SHEAR = 6
+FIELD_SCHMID = 7
+
+ERRORS = {
+    -1: "VPFFT_E_INVALID",
+    -2: "VPFFT_E_NOMEM",
+    -3: "VPFFT_E_DEVICE",
+    -4: "VPFFT_E_DEGENERATE",
+    -5: "VPFFT_E_NONFINITE",
+    -6: "VPFFT_E_UNSUPPORTED",
+    -7: "VPFFT_E_STATE",
+}
+
+
+class VPFFTError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class vpfft_grid_desc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("precision", C.c_int)]
+
+
+class vpfft_material(C.Structure):
+    _fields_ = [
+        ("gamma_dot_base", C.c_double * NSLIP),
+        ("rate_sensitivity", C.c_int * NSLIP),
+        ("tau0", C.c_double * NSLIP),
+        ("tau1", C.c_double * NSLIP),
+        ("theta0", C.c_double * NSLIP),
+        ("theta1", C.c_double * NSLIP),
+        ("hardening", C.c_double * (NSLIP * NSLIP)),
+    ]
+
+
+class vpfft_step_info(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int),
+        ("converged", C.c_int),
+        ("status", C.c_int),
+        ("precision", C.c_int),
+        ("err_stress", C.c_double),
+        ("err_strain_rate", C.c_double),
+        ("avg_stress", C.c_double * 5),
+        ("avg_strain_rate", C.c_double * 5),
+        ("L0", C.c_double * 25),
+        ("max_nr_error", C.c_double),
+        ("nr_steps_last", C.c_double),
+        ("nr_steps_total", C.c_double),
+        ("ms_prologue", C.c_double),
+        ("ms_loop", C.c_double),
+        ("ms_epilogue", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+
+_lib = None
+
+# every symbol include/vpfft.h declares
+EXPORTED = [
+    "vpfft_grid_create", "vpfft_grid_destroy", "vpfft_grid_set_voxel_length",
+    "vpfft_grid_get_voxel_length", "vpfft_grid_set_precision", "vpfft_grid_upload",
+    "vpfft_grid_upload_field", "vpfft_grid_download", "vpfft_run_strain_step",
+    "vpfft_run_vpfft", "vpfft_strain_rate_evolution", "vpfft_solve_constrained_batch",
+    "vpfft_solve_taylor_batch", "vpfft_generate_voronoi", "vpfft_last_error", "vpfft_version",
+    "vpfft_grid_set_profiling", "vpfft_grid_kernel_times", "vpfft_kernel_name",
+]
+
+
+def load_library(path: Optional[str] = None):
+    """Load libvpfft.so (raises when it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} not found: build it with `python -m vpfft_amd.build`")
+    lib = C.CDLL(p)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    lib.vpfft_grid_create.argtypes = [C.POINTER(vpfft_grid_desc), C.POINTER(vpfft_material), C.POINTER(vp)]
+    lib.vpfft_grid_destroy.argtypes = [vp]
+    lib.vpfft_grid_set_voxel_length.argtypes = [vp, C.c_double, C.c_double, C.c_double]
+    lib.vpfft_grid_get_voxel_length.argtypes = [vp, dp]
+    lib.vpfft_grid_set_precision.argtypes = [vp, C.c_int]
+    lib.vpfft_grid_upload.argtypes = [vp, dp, dp, C.c_double, dp]
+    lib.vpfft_grid_upload_field.argtypes = [vp, C.c_int, dp]
+    lib.vpfft_grid_download.argtypes = [vp, C.c_int, dp]
+    lib.vpfft_run_strain_step.argtypes = [vp, dp, C.c_int, C.c_double, C.c_double, dp, C.POINTER(vpfft_step_info)]
+    lib.vpfft_run_vpfft.argtypes = [vp, dp, C.c_int, C.c_int, C.c_double, C.c_double, dp,
+                                    C.POINTER(vpfft_step_info), C.c_char_p]
+    lib.vpfft_strain_rate_evolution.argtypes = [vp, dp, dp, dp]
+    lib.vpfft_solve_constrained_batch.argtypes = [C.c_long, C.POINTER(vpfft_material), dp, dp, dp, dp, dp, dp, dp,
+                                                  dp, dp, C.POINTER(C.c_int)]
+    lib.vpfft_solve_taylor_batch.argtypes = [C.c_long, C.POINTER(vpfft_material), dp, dp, dp, dp, dp]
+    lib.vpfft_generate_voronoi.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, dp,
+                                           C.POINTER(C.c_int32), dp]
+    lib.vpfft_grid_set_profiling.argtypes = [vp, C.c_int]
+    lib.vpfft_grid_kernel_times.argtypes = [vp, dp, C.POINTER(C.c_long), C.c_int]
+    lib.vpfft_kernel_name.argtypes = [C.c_int]
+    lib.vpfft_kernel_name.restype = C.c_char_p
+    lib.vpfft_last_error.restype = C.c_char_p
+    lib.vpfft_version.restype = C.c_char_p
+    for name in EXPORTED:
+        if name not in ("vpfft_last_error", "vpfft_version", "vpfft_kernel_name"):
+            getattr(lib, name).restype = C.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = load_library().vpfft_last_error().decode(errors="replace")
+        raise VPFFTError(rc, msg)
+
+
+def _dptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a, shape=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+# ---------------------------------------------------------------------------
+# material
+# ---------------------------------------------------------------------------
+@dataclass
+class Material:
+    """Slip-system constants (the reference hard-codes them in
+    MaterialGrid::InitializeMemory, Src/MaterialGrid.cpp:322-336)."""
+    gamma_dot_base: Sequence[float] = field(default_factory=lambda: [1.0] * NSLIP)
+    rate_sensitivity: Sequence[int] = field(default_factory=lambda: [10] * NSLIP)
+    tau0: Sequence[float] = field(default_factory=lambda: [11.0] * NSLIP)
+    tau1: Sequence[float] = field(default_factory=lambda: [11.0] * NSLIP)
+    theta0: Sequence[float] = field(default_factory=lambda: [430.0] * NSLIP)
+    theta1: Sequence[float] = field(default_factory=lambda: [110.0] * NSLIP)
+    hardening: np.ndarray = field(default_factory=lambda: np.ones((NSLIP, NSLIP)))
+
+    @classmethod
+    def uniform(cls, gamma_dot_base: float = 1.0, rate_sensitivity: int = 10, **kw) -> "Material":
+        return cls(gamma_dot_base=[float(gamma_dot_base)] * NSLIP,
+                   rate_sensitivity=[int(rate_sensitivity)] * NSLIP, **kw)
+
+    def to_c(self) -> vpfft_material:
+        m = vpfft_material()
+        for i in range(NSLIP):
+            m.gamma_dot_base[i] = float(self.gamma_dot_base[i])
+            m.rate_sensitivity[i] = int(self.rate_sensitivity[i])
+            m.tau0[i] = float(self.tau0[i])
+            m.tau1[i] = float(self.tau1[i])
+            m.theta0[i] = float(self.theta0[i])
+            m.theta1[i] = float(self.theta1[i])
+        h = np.asarray(self.hardening, dtype=np.float64).reshape(NSLIP * NSLIP)
+        for i in range(NSLIP * NSLIP):
+            m.hardening[i] = float(h[i])
+        return m
+
+
+# ---------------------------------------------------------------------------
+# deviatoric basis helpers (Src/LinearAlgebra.h:207-247, 307-360)
+# ---------------------------------------------------------------------------
+def eigen_basis() -> np.ndarray:
+    s2, s6, s3 = 1.0 / np.sqrt(2.0), 1.0 / np.sqrt(6.0), 1.0 / np.sqrt(3.0)
+    B = np.zeros((6, 3, 3))
+    B[0] = s2 * np.array([[-1, 0, 0], [0, 1, 0], [0, 0, 0]])
+    B[1] = s6 * np.array([[-1, 0, 0], [0, -1, 0], [0, 0, 2]])
+    B[2] = s2 * np.array([[0, 0, 0], [0, 0, 1], [0, 1, 0]])
+    B[3] = s2 * np.array([[0, 0, 1], [0, 0, 0], [1, 0, 0]])
+    B[4] = s2 * np.array([[0, 1, 0], [1, 0, 0], [0, 0, 0]])
+    B[5] = s3 * np.eye(3)
+    return B
+
+
+def EigenRep(matrix) -> np.ndarray:
+    """EigenRep(const SMatrix3x3&): the 5 deviatoric components of a 3x3 tensor."""
+    M = np.asarray(matrix, dtype=np.float64).reshape(3, 3)
+    B = eigen_basis()
+    return np.array([np.sum(M * B[a]) for a in range(5)])
+
+
+def ToMatrixRep(v) -> np.ndarray:
+    """EigenRep::ToMatrixRep: 5-vector -> symmetric traceless 3x3."""
+    B = eigen_basis()
+    v = np.asarray(v, dtype=np.float64)
+    return sum(v[a] * B[a] for a in range(5))
+
+
+def generate_voronoi(nx: int, ny: int, nz: int, num_grains: int, seed: int):
+    """Seeded periodic Voronoi polycrystal: (orientation N x 9, grain N, euler G x 3)."""
+    lib = load_library()
+    n = nx * ny * nz
+    R = np.empty((n, 9))
+    grain = np.empty(n, dtype=np.int32)
+    eul = np.empty((num_grains, 3))
+    _check(lib.vpfft_generate_voronoi(nx, ny, nz, num_grains, C.c_uint64(seed), _dptr(R),
+                                      grain.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(eul)))
+    return R, grain, eul
+
+
+# ---------------------------------------------------------------------------
+# MaterialGrid mirror
+# ---------------------------------------------------------------------------
+class MaterialGrid:
+    """GPU strain-step engine with the reference's MaterialGrid interface.
+
+    MaterialGrid(DimX, DimY, DimZ)           Src/MaterialGrid.h:72-78
+    SetVoxelLength(x, y, z)                  :97-102
+    SendDataAndInitialize(R, Stress)         :85-86
+    RunSingleStrainStep(Edot, maxIt, dt, eps) -> <sigma>   :221-223
+    RunVPFFT(Edot, maxIt, steps, dt, eps)    :571-575
+    Stress(), LagrangeMultiplier(), Orientation(), SchmidtTensorList(),
+    CRSS(), DisplacementVariation(), PolarizationField()   :129-185, 338-434
+    NumX/NumY/NumZ, ToIndex                  :121-124, 235-237
+    """
+
+    def __init__(self, DimX: int, DimY: int, DimZ: int, material: Optional[Material] = None,
+                 precision: int = FP64, crss: float = 1.0):
+        self._lib = load_library()
+        self.DimX, self.DimY, self.DimZ = int(DimX), int(DimY), int(DimZ)
+        self.material = material or Material.uniform()
+        self.crss_value = float(crss)
+        desc = vpfft_grid_desc(self.DimX, self.DimY, self.DimZ, int(precision))
+        self._mat_c = self.material.to_c()
+        h = C.c_void_p()
+        _check(self._lib.vpfft_grid_create(C.byref(desc), C.byref(self._mat_c), C.byref(h)))
+        self._h = h
+        self.last_info: Optional[vpfft_step_info] = None
+
+    # ---- lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.vpfft_grid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- geometry
+    @property
+    def N(self) -> int:
+        return self.DimX * self.DimY * self.DimZ
+
+    def NumX(self) -> int:
+        return self.DimX
+
+    def NumY(self) -> int:
+        return self.DimY
+
+    def NumZ(self) -> int:
+        return self.DimZ
+
+    def ToIndex(self, x: int, y: int, z: int) -> int:
+        return (x * self.DimY + y) * self.DimZ + z
+
+    def SetVoxelLength(self, x: float, y: float, z: float):
+        _check(self._lib.vpfft_grid_set_voxel_length(self._h, x, y, z))
+
+    def VoxelLength(self):
+        out = np.empty(3)
+        _check(self._lib.vpfft_grid_get_voxel_length(self._h, _dptr(out)))
+        return out
+
+    def SetProfiling(self, enable: bool = True):
+        """Per-kernel device timing with HIP events on the grid's stream (resets totals)."""
+        _check(self._lib.vpfft_grid_set_profiling(self._h, 1 if enable else 0))
+
+    def KernelTimes(self) -> dict:
+        """{kernel name: (total ms, launches)} since profiling was enabled."""
+        ms = np.zeros(32)
+        cnt = np.zeros(32, dtype=np.int64)
+        n = self._lib.vpfft_grid_kernel_times(self._h, _dptr(ms), cnt.ctypes.data_as(C.POINTER(C.c_long)), 32)
+        if n < 0:
+            _check(n)
+        return {self._lib.vpfft_kernel_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+    def SetPrecision(self, precision: int):
+        _check(self._lib.vpfft_grid_set_precision(self._h, int(precision)))
+
+    # ---- data
+    def SendDataAndInitialize(self, Orientations, Stress=None, CRSS=None):
+        R = _f64(Orientations, (self.N, 9))
+        S = None if Stress is None else _f64(Stress, (self.N, 5))
+        T = None if CRSS is None else _f64(CRSS, (self.N, 12))
+        _check(self._lib.vpfft_grid_upload(self._h, _dptr(R), _dptr(T), self.crss_value, _dptr(S)))
+
+    def _download(self, fid: int, shape) -> np.ndarray:
+        out = np.empty(shape)
+        _check(self._lib.vpfft_grid_download(self._h, fid, _dptr(out)))
+        return out
+
+    def _upload(self, fid: int, arr, shape):
+        a = _f64(arr, shape)
+        _check(self._lib.vpfft_grid_upload_field(self._h, fid, _dptr(a)))
+
+    def Stress(self) -> np.ndarray:
+        return self._download(FIELD_STRESS, (self.N, 5))
+
+    def LagrangeMultiplier(self) -> np.ndarray:
+        return self._download(FIELD_LAGRANGE, (self.N, 5))
+
+    def Orientation(self) -> np.ndarray:
+        return self._download(FIELD_ORIENTATION, (self.N, 9))
+
+    def CRSS(self) -> np.ndarray:
+        return self._download(FIELD_CRSS, (self.N, 12))
+
+    def AccumulatedShear(self) -> np.ndarray:
+        return self._download(FIELD_ACC_SHEAR, (self.N,))
+
+    def SchmidtTensorList(self) -> np.ndarray:
+        return self._download(FIELD_SCHMID, (self.N, 12, 5))
+
+    def PolarizationField(self) -> np.ndarray:
+        return self._download(FIELD_POLARIZATION, (5, self.N))
+
+    def DisplacementVariation(self) -> np.ndarray:
+        return self._download(FIELD_STRAIN_RATE_VARIATION, (5, self.N))
+
+    def SetStress(self, s):
+        self._upload(FIELD_STRESS, s, (self.N, 5))
+
+    def SetLagrangeMultiplier(self, s):
+        self._upload(FIELD_LAGRANGE, s, (self.N, 5))
+
+    def SetCRSS(self, t):
+        self._upload(FIELD_CRSS, t, (self.N, 12))
+
+    def SetOrientation(self, R):
+        self._upload(FIELD_ORIENTATION, R, (self.N, 9))
+
+    def SetAccumulatedShear(self, a):
+        self._upload(FIELD_ACC_SHEAR, a, (self.N,))
+
+    # ---- the hot path
+    def RunSingleStrainStep(self, MacroscopicStrainRate, NumMaxIterations: int, TimeStep: float,
+                            ConvergenceEpsilon: float) -> np.ndarray:
+        e = _f64(MacroscopicStrainRate, (5,))
+        avg = np.empty(5)
+        info = vpfft_step_info()
+        rc = self._lib.vpfft_run_strain_step(self._h, _dptr(e), int(NumMaxIterations), float(TimeStep),
+                                             float(ConvergenceEpsilon), _dptr(avg), C.byref(info))
+        self.last_info = info
+        _check(rc)
+        return avg
+
+    def RunVPFFT(self, MacroscopicStrainRate, NumMaxIterations: int, NumTimeSteps: int, TimeStep: float,
+                 ConvergenceEpsilon: float, stress_strain_path: Optional[str] = None):
+        e = _f64(MacroscopicStrainRate, (5,))
+        avg = np.empty((max(NumTimeSteps, 1), 5))
+        infos = (vpfft_step_info * max(NumTimeSteps, 1))()
+        path = stress_strain_path.encode() if stress_strain_path else None
+        _check(self._lib.vpfft_run_vpfft(self._h, _dptr(e), int(NumMaxIterations), int(NumTimeSteps),
+                                         float(TimeStep), float(ConvergenceEpsilon), _dptr(avg), infos, path))
+        return avg[:NumTimeSteps], [infos[i] for i in range(NumTimeSteps)]
+
+    def StrainRateEvolutionWithFFT(self, L0, phi) -> np.ndarray:
+        """d~ = Re IFFT(-S5 FFT(phi)) / N for a given L0 (MaterialGrid.cpp:624-761)."""
+        L = _f64(L0, (25,))
+        p = _f64(phi, (5, self.N))
+        out = np.empty((5, self.N))
+        _check(self._lib.vpfft_strain_rate_evolution(self._h, _dptr(L), _dptr(p), _dptr(out)))
+        return out
+
+
+def solve_constrained_batch(material: Material, R, crss, L0, edot, stress, lam, dtilde, nr_error=None):
+    """Solvers::SolveConstrainedConstitutiveEquations over n voxels on the GPU."""
+    lib = load_library()
+    R = _f64(R)
+    n = R.reshape(-1, 9).shape[0]
+    crss, L0, edot = _f64(crss, (n, 12)), _f64(L0, (25,)), _f64(edot, (5,))
+    stress, lam, dtilde = _f64(stress, (n, 5)), _f64(lam, (n, 5)), _f64(dtilde, (n, 5))
+    err = np.zeros(n) if nr_error is None else _f64(nr_error, (n,)).copy()
+    out = np.empty((n, 5))
+    steps = np.empty(n, dtype=np.int32)
+    m = material.to_c()
+    _check(lib.vpfft_solve_constrained_batch(n, C.byref(m), _dptr(R), _dptr(crss), _dptr(L0), _dptr(edot),
+                                             _dptr(stress), _dptr(lam), _dptr(dtilde), _dptr(out), _dptr(err),
+                                             steps.ctypes.data_as(C.POINTER(C.c_int))))
+    return out, err, steps
+
+
+def solve_taylor_batch(material: Material, R, crss, edot, stress):
+    """Solvers::SolveConstitutiveEquations (Taylor) over n voxels on the GPU."""
+    lib = load_library()
+    R = _f64(R)
+    n = R.reshape(-1, 9).shape[0]
+    crss, edot, stress = _f64(crss, (n, 12)), _f64(edot, (5,)), _f64(stress, (n, 5))
+    out = np.empty((n, 5))
+    m = material.to_c()
+    _check(lib.vpfft_solve_taylor_batch(n, C.byref(m), _dptr(R), _dptr(crss), _dptr(edot), _dptr(stress),
+                                        _dptr(out)))
+    return out

@@ -1,0 +1,56 @@
+"""Build libvpfft.so (HIP kernels + C ABI) for gfx950 with hipcc.
+
+Usage: python -m vpfft_amd.build [--verbose]
+The library is written next to this file so it travels with the source tree.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libvpfft.so")
+SOURCES = ["capi.hip", "fft_green.hip", "voxel_kernels.hip", "microstructure.cpp"]
+ARCH = os.environ.get("VPFFT_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    return "hipcc"
+
+
+def build(verbose: bool = False) -> str:
+    objs = []
+    common = [
+        "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
+        # no packed-FP32 instructions (v_pk_*_f32) anywhere in the device code
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-ffp-contract=fast-honor-pragmas", "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
+    ]
+    outdir = os.path.join(HERE, "_build")
+    os.makedirs(outdir, exist_ok=True)
+    for src in SOURCES:
+        obj = os.path.join(outdir, src + ".o")
+        cmd = [hipcc()] + common + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if src.endswith(".cpp"):
+            cmd = [hipcc(), "-O3", "-fPIC", "-std=c++17", "-x", "c++", "-c",
+                   os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="--verbose" in sys.argv))

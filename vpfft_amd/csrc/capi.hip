@@ -1,0 +1,840 @@
+// capi.hip -- C ABI of the VPFFT strain-step engine: grid handle, device
+// arena, upload/download and the strain-step orchestration
+// (MaterialGrid::RunSingleStrainStep, Src/MaterialGrid.cpp:897-1084).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vpfft.h"
+#include "capi_internal.h"
+#include "device_common.h"
+#include "fft_green.h"
+#include "voxel_kernels.h"
+
+namespace vpfft {
+
+static thread_local std::string g_err;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess) return set_error(VPFFT_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+__global__ void k_fill(double* p, double v, long n) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = v;
+}
+
+__global__ void k_schmid_aos(const double* __restrict__ R, long n, long offset, long count, double* __restrict__ out) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long)gridDim.x * blockDim.x) {
+    const long v = offset + t;
+    double r[9], m[kSlips][5];
+    for (int e = 0; e < 9; e++) r[e] = R[e * n + v];
+    schmid_from_R(r, m);
+    for (int k = 0; k < kSlips; k++)
+      for (int i = 0; i < 5; i++) out[60 * t + 5 * k + i] = m[k][i];
+  }
+}
+
+static int fill_blocks(long n) {
+  long b = (n + 255) / 256;
+  return (int)std::max(1L, std::min(b, 8192L));
+}
+
+enum KernelId {
+  K_PROLOGUE = 0, K_L0_FINALIZE, K_Z_R2C, K_Y_FWD, K_X_GREEN, K_Y_INV, K_Z_C2R, K_NR, K_ITER_FINALIZE,
+  K_EPILOGUE, kNumKernels
+};
+static const char* kKernelNames[kNumKernels] = {"prologue", "L0_finalize", "fft_z_r2c", "fft_y_fwd",
+                                                "fft_x_green", "fft_y_inv", "fft_z_c2r", "nr_update",
+                                                "iter_finalize", "epilogue"};
+
+struct Grid {
+  int nx = 0, ny = 0, nz = 0, nzh = 0;
+  long n = 0, nc = 0;
+  int prec = 0;
+  double lx = 1, ly = 1, lz = 1;
+  vpfft_material mat{};
+  hipStream_t st = nullptr;
+  double *R = nullptr, *crss = nullptr, *acc = nullptr, *sigma = nullptr, *lambda = nullptr;
+  void *phi = nullptr, *dtil = nullptr, *spec = nullptr;  // sized for FP64
+  void* tw64[3] = {nullptr, nullptr, nullptr};
+  void* tw32[3] = {nullptr, nullptr, nullptr};
+  AxisPlan plan[3];
+  FftGeom geom{};
+  StepConst* dC = nullptr;
+  GreenConst<double>* dG64 = nullptr;
+  GreenConst<float>* dG32 = nullptr;
+  double* partial = nullptr;
+  IterRecord* drec = nullptr;
+  int rec_cap = 0;
+  PrologueRecord* dprol = nullptr;
+  StepConst* hC = nullptr;
+  IterRecord* hrec = nullptr;
+  PrologueRecord* hprol = nullptr;
+  double* staging = nullptr;
+  long staging_elems = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool uploaded = false;
+  int phi_prec = 0, dtil_prec = 0;  // precision the field buffers currently hold
+  // per-kernel timing with HIP events on the grid's stream
+  bool profile = false;
+  hipEvent_t kev[kNumKernels][2] = {};
+  bool kev_used[kNumKernels] = {};
+  double kms[kNumKernels] = {};
+  long kcount[kNumKernels] = {};
+  std::vector<void*> dev_allocs;
+  std::vector<void*> host_allocs;
+};
+
+static int dalloc(Grid* g, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_error(VPFFT_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  }
+  g->dev_allocs.push_back(*p);
+  return VPFFT_OK;
+}
+static int halloc(Grid* g, void** p, size_t bytes) {
+  hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_error(VPFFT_E_NOMEM, "hipHostMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  }
+  g->host_allocs.push_back(*p);
+  return VPFFT_OK;
+}
+
+static void grid_free(Grid* g) {
+  if (!g) return;
+  if (g->st) (void)hipStreamSynchronize(g->st);
+  for (void* p : g->dev_allocs) (void)hipFree(p);
+  for (void* p : g->host_allocs) (void)hipHostFree(p);
+  for (auto& e : g->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& pr : g->kev)
+    for (auto& e : pr)
+      if (e) (void)hipEventDestroy(e);
+  if (g->st) (void)hipStreamDestroy(g->st);
+  delete g;
+}
+
+static int make_twiddles(Grid* g, int axis, int n) {
+  std::vector<double2> t64(n);
+  std::vector<float2> t32(n);
+  for (int t = 0; t < n; t++) {
+    const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)n;
+    t64[t].x = (double)cosl(a);
+    t64[t].y = (double)sinl(a);
+    t32[t].x = (float)t64[t].x;
+    t32[t].y = (float)t64[t].y;
+  }
+  int rc;
+  if ((rc = dalloc(g, &g->tw64[axis], sizeof(double2) * n))) return rc;
+  if ((rc = dalloc(g, &g->tw32[axis], sizeof(float2) * n))) return rc;
+  HIP_TRY(hipMemcpy(g->tw64[axis], t64.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(g->tw32[axis], t32.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+  return VPFFT_OK;
+}
+
+static int ensure_records(Grid* g, int cap) {
+  if (cap <= g->rec_cap) return VPFFT_OK;
+  int newcap = std::max(cap, 2 * g->rec_cap);
+  void *d = nullptr, *h = nullptr;
+  int rc;
+  if ((rc = dalloc(g, &d, sizeof(IterRecord) * newcap))) return rc;
+  if ((rc = halloc(g, &h, sizeof(IterRecord) * newcap))) return rc;
+  g->drec = (IterRecord*)d;
+  g->hrec = (IterRecord*)h;
+  g->rec_cap = newcap;
+  return VPFFT_OK;
+}
+
+// basis diagonal of EigenRep::ToMatrixRep (LinearAlgebra.h:352-360)
+static void edot_diag(const double e[5], double d[3]) {
+  const double s2 = 1.0 / std::sqrt(2.0), s6 = 1.0 / std::sqrt(6.0);
+  d[0] = 0.0 + (-1.0 * s2) * e[0] + (-1.0 * s6) * e[1];
+  d[1] = 0.0 + (1.0 * s2) * e[0] + (-1.0 * s6) * e[1];
+  d[2] = 0.0 + (0.0 * s2) * e[0] + (2.0 * s6) * e[1];
+}
+
+static void fill_host_const(Grid* g, const double edot[5], double dt) {
+  StepConst& C = *g->hC;
+  std::memset(&C, 0, sizeof C);
+  for (int i = 0; i < 5; i++) C.edot[i] = edot[i];
+  int nmin = g->mat.rate_sensitivity[0];
+  C.uniform_n = 1;
+  for (int k = 0; k < kSlips; k++) {
+    C.gdot0[k] = g->mat.gamma_dot_base[k];
+    C.nexp[k] = g->mat.rate_sensitivity[k];
+    if (C.nexp[k] != C.nexp[0]) C.uniform_n = 0;
+    nmin = std::min(nmin, C.nexp[k]);
+    C.tau0[k] = g->mat.tau0[k];
+    C.tau1[k] = g->mat.tau1[k];
+    C.theta0[k] = g->mat.theta0[k];
+    C.theta1[k] = g->mat.theta1[k];
+  }
+  for (int e = 0; e < kSlips * kSlips; e++) C.hmat[e] = g->mat.hardening[e];
+  // MaterialGrid.cpp:162-166
+  double ee = 0;
+  for (int i = 0; i < 5; i++) ee += edot[i] * edot[i];
+  double imposed = 2.0 * std::pow(ee / C.gdot0[0], 1.0 / (double)(float)nmin);
+  C.imposed_max_rss = std::max(2.0, imposed);
+  C.inv_n = 1.0 / (double)g->n;
+  C.dt = dt;
+  C.kscale[0] = 1.0 / ((double)g->nx * g->lx);
+  C.kscale[1] = 1.0 / ((double)g->ny * g->ly);
+  C.kscale[2] = 1.0 / ((double)g->nz * g->lz);
+  C.nx = g->nx;
+  C.ny = g->ny;
+  C.nz = g->nz;
+  C.nzh = g->nzh;
+}
+
+// Bracket one kernel launch with events when profiling is on.
+static inline void kbegin(Grid* g, int k) {
+  if (g->profile) {
+    (void)hipEventRecord(g->kev[k][0], g->st);
+    g->kev_used[k] = true;
+  }
+}
+static inline void kend(Grid* g, int k) {
+  if (g->profile) (void)hipEventRecord(g->kev[k][1], g->st);
+}
+// After a stream synchronisation: fold the recorded intervals into the totals.
+static void kcollect(Grid* g) {
+  if (!g->profile) return;
+  for (int k = 0; k < kNumKernels; k++) {
+    if (!g->kev_used[k]) continue;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, g->kev[k][0], g->kev[k][1]) == hipSuccess) {
+      g->kms[k] += ms;
+      g->kcount[k] += 1;
+    }
+    g->kev_used[k] = false;
+  }
+}
+
+static int run_fft_green(Grid* g, int prec) {
+  void* const* tw = prec == 0 ? g->tw64 : g->tw32;
+  const void* gc = prec == 0 ? (const void*)g->dG64 : (const void*)g->dG32;
+  kbegin(g, K_Z_R2C);
+  HIP_TRY(launch_z_r2c(prec, g->phi, g->spec, g->geom, g->plan[2], tw[2], g->st));
+  kend(g, K_Z_R2C);
+  kbegin(g, K_Y_FWD);
+  HIP_TRY(launch_y(prec, g->spec, g->geom, g->plan[1], tw[1], -1, g->st));
+  kend(g, K_Y_FWD);
+  kbegin(g, K_X_GREEN);
+  HIP_TRY(launch_x_green(prec, g->spec, g->geom, g->plan[0], tw[0], gc, g->st));
+  kend(g, K_X_GREEN);
+  kbegin(g, K_Y_INV);
+  HIP_TRY(launch_y(prec, g->spec, g->geom, g->plan[1], tw[1], +1, g->st));
+  kend(g, K_Y_INV);
+  kbegin(g, K_Z_C2R);
+  HIP_TRY(launch_z_c2r(prec, g->spec, g->dtil, g->geom, g->plan[2], tw[2], 1.0 / (double)g->n, g->st));
+  kend(g, K_Z_C2R);
+  g->dtil_prec = prec;
+  return VPFFT_OK;
+}
+
+// host AoS (count x width) -> device SoA (width x n), through the staging buffer
+static int upload_aos(Grid* g, const double* host, double* dev, int width) {
+  const long chunk = g->staging_elems / width;
+  for (long off = 0; off < g->n; off += chunk) {
+    const long cnt = std::min(chunk, g->n - off);
+    HIP_TRY(hipMemcpyAsync(g->staging, host + off * width, sizeof(double) * cnt * width, hipMemcpyHostToDevice, g->st));
+    HIP_TRY(launch_aos_to_soa(g->staging, dev, cnt, width, g->n, off, g->st));
+  }
+  HIP_TRY(hipStreamSynchronize(g->st));
+  return VPFFT_OK;
+}
+static int download_aos(Grid* g, const double* dev, double* host, int width) {
+  const long chunk = g->staging_elems / width;
+  for (long off = 0; off < g->n; off += chunk) {
+    const long cnt = std::min(chunk, g->n - off);
+    HIP_TRY(launch_soa_to_aos(dev, g->staging, cnt, width, g->n, off, g->st));
+    HIP_TRY(hipMemcpyAsync(host + off * width, g->staging, sizeof(double) * cnt * width, hipMemcpyDeviceToHost, g->st));
+    HIP_TRY(hipStreamSynchronize(g->st));
+  }
+  return VPFFT_OK;
+}
+// 5 x N field in precision prec <-> host doubles
+static int upload_field5(Grid* g, const double* host, void* dev, int prec) {
+  if (prec == 0) {
+    HIP_TRY(hipMemcpyAsync(dev, host, sizeof(double) * 5 * g->n, hipMemcpyHostToDevice, g->st));
+  } else {
+    const long chunk = g->staging_elems;
+    for (long off = 0; off < 5 * g->n; off += chunk) {
+      const long cnt = std::min(chunk, 5 * g->n - off);
+      HIP_TRY(hipMemcpyAsync(g->staging, host + off, sizeof(double) * cnt, hipMemcpyHostToDevice, g->st));
+      HIP_TRY(launch_convert(1, (float*)dev + off, 0, g->staging, cnt, g->st));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(g->st));
+  return VPFFT_OK;
+}
+static int download_field5(Grid* g, const void* dev, double* host, int prec) {
+  if (prec == 0) {
+    HIP_TRY(hipMemcpyAsync(host, dev, sizeof(double) * 5 * g->n, hipMemcpyDeviceToHost, g->st));
+    HIP_TRY(hipStreamSynchronize(g->st));
+  } else {
+    const long chunk = g->staging_elems;
+    for (long off = 0; off < 5 * g->n; off += chunk) {
+      const long cnt = std::min(chunk, 5 * g->n - off);
+      HIP_TRY(launch_convert(0, g->staging, 1, (const float*)dev + off, cnt, g->st));
+      HIP_TRY(hipMemcpyAsync(host + off, g->staging, sizeof(double) * cnt, hipMemcpyDeviceToHost, g->st));
+      HIP_TRY(hipStreamSynchronize(g->st));
+    }
+  }
+  return VPFFT_OK;
+}
+
+static int check_material(const vpfft_material* m) {
+  for (int k = 0; k < kSlips; k++) {
+    if (!(m->gamma_dot_base[k] > 0) || !std::isfinite(m->gamma_dot_base[k]))
+      return set_error(VPFFT_E_INVALID, "gamma_dot_base[%d] must be positive and finite", k);
+    if (m->rate_sensitivity[k] < 1 || m->rate_sensitivity[k] > 200)
+      return set_error(VPFFT_E_INVALID, "rate_sensitivity[%d]=%d out of range [1,200]", k, m->rate_sensitivity[k]);
+  }
+  return VPFFT_OK;
+}
+
+static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid** out) {
+  if (!d || !mat || !out) return set_error(VPFFT_E_INVALID, "null argument");
+  *out = nullptr;
+  if (d->nx < 2 || d->ny < 2 || d->nz < 2 || d->nx > 4096 || d->ny > 4096 || d->nz > 4096)
+    return set_error(VPFFT_E_INVALID, "grid edges must be in [2, 4096]");
+  if (d->precision != VPFFT_FP64 && d->precision != VPFFT_MIXED)
+    return set_error(VPFFT_E_INVALID, "precision must be VPFFT_FP64 or VPFFT_MIXED");
+  int rc = check_material(mat);
+  if (rc) return rc;
+  Grid* g = new Grid();
+  g->nx = d->nx;
+  g->ny = d->ny;
+  g->nz = d->nz;
+  g->nzh = d->nz / 2 + 1;
+  g->n = (long)d->nx * d->ny * d->nz;
+  g->nc = (long)d->nx * d->ny * g->nzh;
+  g->prec = d->precision;
+  g->mat = *mat;
+  const int dims[3] = {g->nx, g->ny, g->nz};
+  for (int a = 0; a < 3; a++)
+    if (!make_axis_plan(dims[a], &g->plan[a])) {
+      grid_free(g);
+      return set_error(VPFFT_E_UNSUPPORTED, "grid edge %d has a prime factor other than 2, 3, 5, 7", dims[a]);
+    }
+  // LDS limits of the FFT passes (2 ping-pong buffers of one pencil batch in FP64)
+  if ((long)3 * g->nz * 16 * 2 > 160 * 1024 || (long)g->ny * 16 * 2 > 160 * 1024 ||
+      (long)g->nx * 16 * 2 > 160 * 1024) {
+    grid_free(g);
+    return set_error(VPFFT_E_UNSUPPORTED, "grid edge too long for the LDS-resident FFT");
+  }
+  g->geom.n = g->n;
+  g->geom.nc = g->nc;
+  g->geom.nrows = (long)g->nx * g->ny;
+  g->geom.nx = g->nx;
+  g->geom.ny = g->ny;
+  g->geom.nz = g->nz;
+  g->geom.nzh = g->nzh;
+  hipError_t e = hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    grid_free(g);
+    return set_error(VPFFT_E_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  for (auto& ev : g->ev)
+    if (hipEventCreate(&ev) != hipSuccess) {
+      grid_free(g);
+      return set_error(VPFFT_E_DEVICE, "hipEventCreate failed");
+    }
+  for (auto& pr : g->kev)
+    for (auto& ev : pr)
+      if (hipEventCreate(&ev) != hipSuccess) {
+        grid_free(g);
+        return set_error(VPFFT_E_DEVICE, "hipEventCreate failed");
+      }
+  const long n = g->n;
+  void* p;
+#define ALLOC(field, bytes)                          \
+  if ((rc = dalloc(g, &p, (bytes)))) {               \
+    grid_free(g);                                    \
+    return rc;                                       \
+  }                                                  \
+  field = (decltype(field))p;
+  ALLOC(g->R, sizeof(double) * 9 * n);
+  ALLOC(g->crss, sizeof(double) * 12 * n);
+  ALLOC(g->acc, sizeof(double) * n);
+  ALLOC(g->sigma, sizeof(double) * 5 * n);
+  ALLOC(g->lambda, sizeof(double) * 5 * n);
+  ALLOC(g->phi, sizeof(double) * 5 * n);
+  ALLOC(g->dtil, sizeof(double) * 5 * n);
+  ALLOC(g->spec, sizeof(double2) * 5 * g->nc);
+  ALLOC(g->dC, sizeof(StepConst));
+  ALLOC(g->dG64, sizeof(GreenConst<double>));
+  ALLOC(g->dG32, sizeof(GreenConst<float>));
+  ALLOC(g->partial, sizeof(double) * 4096 * 18);
+  ALLOC(g->dprol, sizeof(PrologueRecord));
+  g->staging_elems = std::min<long>(60L * n, 8L << 20);
+  ALLOC(g->staging, sizeof(double) * g->staging_elems);
+#undef ALLOC
+  if ((rc = halloc(g, &p, sizeof(StepConst)))) { grid_free(g); return rc; }
+  g->hC = (StepConst*)p;
+  if ((rc = halloc(g, &p, sizeof(PrologueRecord)))) { grid_free(g); return rc; }
+  g->hprol = (PrologueRecord*)p;
+  if ((rc = ensure_records(g, 128))) { grid_free(g); return rc; }
+  for (int a = 0; a < 3; a++)
+    if ((rc = make_twiddles(g, a, dims[a]))) { grid_free(g); return rc; }
+  *out = g;
+  return VPFFT_OK;
+}
+
+static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, double tol, double avg[5],
+                       vpfft_step_info* info) {
+  if (!g->uploaded) return set_error(VPFFT_E_STATE, "run_strain_step before upload");
+  if (max_iter < 1) return set_error(VPFFT_E_INVALID, "max_iter must be >= 1");
+  if (!(tol >= 0) || !std::isfinite(dt)) return set_error(VPFFT_E_INVALID, "bad tolerance or time step");
+  double ee = 0;
+  for (int i = 0; i < 5; i++) {
+    if (!std::isfinite(edot[i])) return set_error(VPFFT_E_NONFINITE, "non-finite strain rate");
+    ee += edot[i] * edot[i];
+  }
+  if (!(ee > 0)) return set_error(VPFFT_E_DEGENERATE, "zero strain rate: max resolved shear stress < 1e-10");
+  int rc = ensure_records(g, max_iter);
+  if (rc) return rc;
+  const int prec = g->prec;
+  const long n = g->n;
+  const int nb = point_blocks(n);
+  fill_host_const(g, edot, dt);
+  HIP_TRY(hipMemcpyAsync(g->dC, g->hC, sizeof(StepConst), hipMemcpyHostToDevice, g->st));
+  HIP_TRY(hipEventRecord(g->ev[0], g->st));
+  // prologue: MaterialGrid.cpp:903-913
+  kbegin(g, K_PROLOGUE);
+  HIP_TRY(launch_prologue(prec, n, g->R, g->crss, g->dC, g->phi, g->sigma, g->lambda, g->partial, g->st));
+  kend(g, K_PROLOGUE);
+  g->phi_prec = prec;
+  kbegin(g, K_L0_FINALIZE);
+  HIP_TRY(launch_L0_finalize(n, nb, g->partial, g->dC, g->dG64, g->dG32, g->dprol, g->st));
+  kend(g, K_L0_FINALIZE);
+  HIP_TRY(hipMemcpyAsync(g->hprol, g->dprol, sizeof(PrologueRecord), hipMemcpyDeviceToHost, g->st));
+  HIP_TRY(hipEventRecord(g->ev[1], g->st));
+  HIP_TRY(hipStreamSynchronize(g->st));
+  kcollect(g);
+  if (g->hprol->degenerate > 0)
+    return set_error(VPFFT_E_DEGENERATE, "%.0f voxels with max resolved shear stress < 1e-10 (initialization)",
+                     g->hprol->degenerate);
+  // fixed-point loop: MaterialGrid.cpp:930-1068
+  int it = 0;
+  double nr_total = 0;
+  for (;;) {
+    if ((rc = run_fft_green(g, prec))) return rc;
+    kbegin(g, K_NR);
+    HIP_TRY(launch_nr(prec, n, g->dtil, g->phi, g->sigma, g->lambda, g->R, g->crss, g->dC, g->partial, g->st));
+    kend(g, K_NR);
+    g->phi_prec = prec;
+    kbegin(g, K_ITER_FINALIZE);
+    HIP_TRY(launch_iter_finalize(n, nb, g->partial, g->dC, tol, max_iter, it, g->drec, g->st));
+    kend(g, K_ITER_FINALIZE);
+    HIP_TRY(hipMemcpyAsync(&g->hrec[it], &g->drec[it], sizeof(IterRecord), hipMemcpyDeviceToHost, g->st));
+    HIP_TRY(hipStreamSynchronize(g->st));
+    kcollect(g);
+    const IterRecord& r = g->hrec[it];
+    nr_total += r.nr_steps;
+    if (r.degenerate > 0)
+      return set_error(VPFFT_E_DEGENERATE, "%.0f voxels with max resolved shear stress < 1e-10 (iteration %d)",
+                       r.degenerate, it + 1);
+    it++;
+    if (r.done) break;
+  }
+  HIP_TRY(hipEventRecord(g->ev[2], g->st));
+  const IterRecord& last = g->hrec[it - 1];
+  // epilogue: MaterialGrid.cpp:1074-1079 (skipped for dt <= 0, where the
+  // reference's hardening divides 0 by 0)
+  if (dt > 0) {
+    kbegin(g, K_EPILOGUE);
+    HIP_TRY(launch_epilogue(n, g->R, g->crss, g->acc, g->sigma, g->dC, g->st));
+    kend(g, K_EPILOGUE);
+    double d[3];
+    edot_diag(edot, d);
+    g->lx *= (1.0 + d[0] * dt);
+    g->ly *= (1.0 + d[1] * dt);
+    g->lz *= (1.0 + d[2] * dt);
+  }
+  HIP_TRY(hipEventRecord(g->ev[3], g->st));
+  HIP_TRY(hipStreamSynchronize(g->st));
+  kcollect(g);
+  bool finite = true;
+  for (int i = 0; i < 5; i++) finite = finite && std::isfinite(last.avg_stress[i]);
+  finite = finite && std::isfinite(last.err_stress) && std::isfinite(last.err_strain_rate);
+  if (avg)
+    for (int i = 0; i < 5; i++) avg[i] = last.avg_stress[i];
+  if (info) {
+    std::memset(info, 0, sizeof *info);
+    info->iterations = it;
+    info->converged = (last.err_stress <= tol && last.err_strain_rate <= tol) ? 1 : 0;
+    info->precision = prec;
+    info->err_stress = last.err_stress;
+    info->err_strain_rate = last.err_strain_rate;
+    for (int i = 0; i < 5; i++) {
+      info->avg_stress[i] = last.avg_stress[i];
+      info->avg_strain_rate[i] = last.avg_strain_rate[i];
+    }
+    for (int e = 0; e < 25; e++) info->L0[e] = g->hprol->L0[e];
+    info->max_nr_error = last.max_nr_error;
+    info->nr_steps_last = last.nr_steps;
+    info->nr_steps_total = nr_total;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, g->ev[0], g->ev[1]);
+    info->ms_prologue = ms;
+    (void)hipEventElapsedTime(&ms, g->ev[1], g->ev[2]);
+    info->ms_loop = ms;
+    (void)hipEventElapsedTime(&ms, g->ev[2], g->ev[3]);
+    info->ms_epilogue = ms;
+    info->status = finite ? VPFFT_OK : VPFFT_E_NONFINITE;
+  }
+  if (!finite) return set_error(VPFFT_E_NONFINITE, "non-finite average stress or error norm");
+  return VPFFT_OK;
+}
+
+}  // namespace vpfft
+
+using vpfft::Grid;
+using vpfft::set_error;
+
+static Grid* as_grid(void* p) { return static_cast<Grid*>(p); }
+
+extern "C" {
+
+const char* vpfft_last_error(void) { return vpfft::g_err.c_str(); }
+const char* vpfft_version(void) { return "vpfft-mi355x 0.1 (gfx950)"; }
+
+int vpfft_grid_create(const vpfft_grid_desc* desc, const vpfft_material* mat, void** out_grid) {
+  Grid* g = nullptr;
+  int rc = vpfft::grid_create(desc, mat, &g);
+  if (out_grid) *out_grid = rc == VPFFT_OK ? (void*)g : nullptr;
+  return rc;
+}
+
+int vpfft_grid_destroy(void* grid) {
+  vpfft::grid_free(as_grid(grid));
+  return VPFFT_OK;
+}
+
+int vpfft_grid_set_voxel_length(void* grid, double lx, double ly, double lz) {
+  Grid* g = as_grid(grid);
+  if (!g) return set_error(VPFFT_E_INVALID, "null grid");
+  if (!(lx > 0 && ly > 0 && lz > 0)) return set_error(VPFFT_E_INVALID, "voxel lengths must be positive");
+  g->lx = lx;
+  g->ly = ly;
+  g->lz = lz;
+  return VPFFT_OK;
+}
+
+int vpfft_grid_get_voxel_length(void* grid, double out[3]) {
+  Grid* g = as_grid(grid);
+  if (!g || !out) return set_error(VPFFT_E_INVALID, "null argument");
+  out[0] = g->lx;
+  out[1] = g->ly;
+  out[2] = g->lz;
+  return VPFFT_OK;
+}
+
+int vpfft_grid_set_precision(void* grid, int precision) {
+  Grid* g = as_grid(grid);
+  if (!g) return set_error(VPFFT_E_INVALID, "null grid");
+  if (precision != VPFFT_FP64 && precision != VPFFT_MIXED) return set_error(VPFFT_E_INVALID, "bad precision");
+  g->prec = precision;
+  return VPFFT_OK;
+}
+
+int vpfft_grid_set_profiling(void* grid, int enable) {
+  Grid* g = as_grid(grid);
+  if (!g) return set_error(VPFFT_E_INVALID, "null grid");
+  g->profile = enable != 0;
+  for (int k = 0; k < vpfft::kNumKernels; k++) {
+    g->kms[k] = 0;
+    g->kcount[k] = 0;
+    g->kev_used[k] = false;
+  }
+  return VPFFT_OK;
+}
+
+int vpfft_grid_kernel_times(void* grid, double* total_ms, long* launches, int capacity) {
+  Grid* g = as_grid(grid);
+  if (!g || !total_ms || !launches) return set_error(VPFFT_E_INVALID, "null argument");
+  const int n = std::min(capacity, (int)vpfft::kNumKernels);
+  for (int k = 0; k < n; k++) {
+    total_ms[k] = g->kms[k];
+    launches[k] = g->kcount[k];
+  }
+  return vpfft::kNumKernels;
+}
+
+const char* vpfft_kernel_name(int id) {
+  return (id >= 0 && id < vpfft::kNumKernels) ? vpfft::kKernelNames[id] : "";
+}
+
+int vpfft_grid_upload(void* grid, const double* orientation, const double* crss, double crss_value,
+                      const double* stress) {
+  Grid* g = as_grid(grid);
+  if (!g || !orientation) return set_error(VPFFT_E_INVALID, "null grid or orientation");
+  int rc;
+  if ((rc = vpfft::upload_aos(g, orientation, g->R, 9))) return rc;
+  if (crss) {
+    if ((rc = vpfft::upload_aos(g, crss, g->crss, 12))) return rc;
+  } else {
+    if (!(crss_value > 0)) return set_error(VPFFT_E_INVALID, "crss_value must be positive");
+    hipLaunchKernelGGL(vpfft::k_fill, dim3(vpfft::fill_blocks(12 * g->n)), dim3(256), 0, g->st, g->crss, crss_value,
+                       12 * g->n);
+  }
+  if (stress) {
+    if ((rc = vpfft::upload_aos(g, stress, g->sigma, 5))) return rc;
+  } else {
+    hipLaunchKernelGGL(vpfft::k_fill, dim3(vpfft::fill_blocks(5 * g->n)), dim3(256), 0, g->st, g->sigma, 0.0,
+                       5 * g->n);
+  }
+  hipLaunchKernelGGL(vpfft::k_fill, dim3(vpfft::fill_blocks(g->n)), dim3(256), 0, g->st, g->acc, 0.0, g->n);
+  hipError_t e = hipStreamSynchronize(g->st);
+  if (e != hipSuccess) return set_error(VPFFT_E_DEVICE, "upload: %s", hipGetErrorString(e));
+  g->uploaded = true;
+  return VPFFT_OK;
+}
+
+int vpfft_grid_upload_field(void* grid, int field, const double* host) {
+  Grid* g = as_grid(grid);
+  if (!g || !host) return set_error(VPFFT_E_INVALID, "null argument");
+  switch (field) {
+    case VPFFT_FIELD_ORIENTATION: return vpfft::upload_aos(g, host, g->R, 9);
+    case VPFFT_FIELD_CRSS: return vpfft::upload_aos(g, host, g->crss, 12);
+    case VPFFT_FIELD_STRESS: return vpfft::upload_aos(g, host, g->sigma, 5);
+    case VPFFT_FIELD_LAGRANGE: return vpfft::upload_aos(g, host, g->lambda, 5);
+    case VPFFT_FIELD_POLARIZATION: g->phi_prec = g->prec; return vpfft::upload_field5(g, host, g->phi, g->prec);
+    case VPFFT_FIELD_ACC_SHEAR: {
+      hipError_t e = hipMemcpy(g->acc, host, sizeof(double) * g->n, hipMemcpyHostToDevice);
+      return e == hipSuccess ? VPFFT_OK : set_error(VPFFT_E_DEVICE, "upload acc shear: %s", hipGetErrorString(e));
+    }
+    default: return set_error(VPFFT_E_INVALID, "field %d cannot be uploaded", field);
+  }
+}
+
+int vpfft_grid_download(void* grid, int field, double* host) {
+  Grid* g = as_grid(grid);
+  if (!g || !host) return set_error(VPFFT_E_INVALID, "null argument");
+  switch (field) {
+    case VPFFT_FIELD_ORIENTATION: return vpfft::download_aos(g, g->R, host, 9);
+    case VPFFT_FIELD_CRSS: return vpfft::download_aos(g, g->crss, host, 12);
+    case VPFFT_FIELD_STRESS: return vpfft::download_aos(g, g->sigma, host, 5);
+    case VPFFT_FIELD_LAGRANGE: return vpfft::download_aos(g, g->lambda, host, 5);
+    case VPFFT_FIELD_POLARIZATION: return vpfft::download_field5(g, g->phi, host, g->phi_prec);
+    case VPFFT_FIELD_STRAIN_RATE_VARIATION: return vpfft::download_field5(g, g->dtil, host, g->dtil_prec);
+    case VPFFT_FIELD_ACC_SHEAR: {
+      hipError_t e = hipMemcpy(host, g->acc, sizeof(double) * g->n, hipMemcpyDeviceToHost);
+      return e == hipSuccess ? VPFFT_OK : set_error(VPFFT_E_DEVICE, "download acc shear: %s", hipGetErrorString(e));
+    }
+    case VPFFT_FIELD_SCHMID: {
+      const long chunk = g->staging_elems / 60;
+      for (long off = 0; off < g->n; off += chunk) {
+        const long cnt = std::min(chunk, g->n - off);
+        hipLaunchKernelGGL(vpfft::k_schmid_aos, dim3(vpfft::fill_blocks(cnt)), dim3(256), 0, g->st, g->R, g->n, off,
+                           cnt, g->staging);
+        hipError_t e = hipMemcpyAsync(host + 60 * off, g->staging, sizeof(double) * 60 * cnt, hipMemcpyDeviceToHost,
+                                      g->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(g->st);
+        if (e != hipSuccess) return set_error(VPFFT_E_DEVICE, "download schmid: %s", hipGetErrorString(e));
+      }
+      return VPFFT_OK;
+    }
+    default: return set_error(VPFFT_E_INVALID, "unknown field %d", field);
+  }
+}
+
+int vpfft_run_strain_step(void* grid, const double edot[5], int max_iter, double dt, double tol,
+                          double avg_stress[5], vpfft_step_info* info) {
+  Grid* g = as_grid(grid);
+  if (!g || !edot) return set_error(VPFFT_E_INVALID, "null argument");
+  return vpfft::strain_step(g, edot, max_iter, dt, tol, avg_stress, info);
+}
+
+int vpfft_run_vpfft(void* grid, const double edot[5], int max_iter, int num_steps, double dt, double tol,
+                    double* avg_stress_out, vpfft_step_info* infos, const char* stress_strain_path) {
+  Grid* g = as_grid(grid);
+  if (!g || !edot || num_steps < 0) return set_error(VPFFT_E_INVALID, "bad argument");
+  FILE* os = nullptr;
+  if (stress_strain_path) {
+    os = std::fopen(stress_strain_path, "w");
+    if (!os) return set_error(VPFFT_E_INVALID, "cannot open %s", stress_strain_path);
+  }
+  double E[5] = {0, 0, 0, 0, 0};
+  for (int s = 0; s < num_steps; s++) {
+    double avg[5];
+    vpfft_step_info info;
+    int rc = vpfft::strain_step(g, edot, max_iter, dt, tol, avg, &info);
+    if (rc) {
+      if (os) std::fclose(os);
+      return rc;
+    }
+    if (avg_stress_out)
+      for (int i = 0; i < 5; i++) avg_stress_out[5 * s + i] = avg[i];
+    if (infos) infos[s] = info;
+    for (int i = 0; i < 5; i++) E[i] += edot[i] * dt;  // MaterialGrid.cpp:1217
+    if (os) {  // MaterialGrid.cpp:1221-1225 (ostream default format == %g)
+      double nE = 0, nS = 0;
+      for (int i = 0; i < 5; i++) { nE += E[i] * E[i]; nS += avg[i] * avg[i]; }
+      std::fprintf(os, "%d %g %g %g %g %g  %g %g %g %g %g  %g %g \n", s, E[0], E[1], E[2], E[3], E[4], avg[0], avg[1],
+                   avg[2], avg[3], avg[4], std::sqrt(nE), std::sqrt(nS));
+    }
+  }
+  if (os) std::fclose(os);
+  return VPFFT_OK;
+}
+
+int vpfft_strain_rate_evolution(void* grid, const double L0[25], const double* phi_in, double* dtilde_out) {
+  Grid* g = as_grid(grid);
+  if (!g || !L0 || !phi_in || !dtilde_out) return set_error(VPFFT_E_INVALID, "null argument");
+  const double e0[5] = {1, 0, 0, 0, 0};
+  vpfft::fill_host_const(g, e0, 0.0);
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) g->hC->L0[j * 5 + i] = 0.5 * (L0[j * 5 + i] + L0[i * 5 + j]);
+  hipError_t e = hipMemcpyAsync(g->dC, g->hC, sizeof(vpfft::StepConst), hipMemcpyHostToDevice, g->st);
+  if (e != hipSuccess) return set_error(VPFFT_E_DEVICE, "%s", hipGetErrorString(e));
+  e = vpfft::launch_derive_from_L0(g->dC, g->dG64, g->dG32, g->st);
+  if (e != hipSuccess) return set_error(VPFFT_E_DEVICE, "%s", hipGetErrorString(e));
+  int rc = vpfft::upload_field5(g, phi_in, g->phi, g->prec);
+  if (rc) return rc;
+  g->phi_prec = g->prec;
+  if ((rc = vpfft::run_fft_green(g, g->prec))) return rc;
+  e = hipStreamSynchronize(g->st);
+  if (e != hipSuccess) return set_error(VPFFT_E_DEVICE, "%s", hipGetErrorString(e));
+  return vpfft::download_field5(g, g->dtil, dtilde_out, g->dtil_prec);
+}
+
+// ---- per-voxel solver batches --------------------------------------------
+namespace {
+struct BatchBufs {
+  std::vector<void*> ptrs;
+  ~BatchBufs() {
+    for (void* p : ptrs) hipFree(p);
+  }
+  void* get(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    return p;
+  }
+};
+
+int batch_const(const vpfft_material* mat, const double L0[25], const double edot[5], vpfft::StepConst* hc) {
+  std::memset(hc, 0, sizeof *hc);
+  int nmin = mat->rate_sensitivity[0];
+  hc->uniform_n = 1;
+  for (int k = 0; k < vpfft::kSlips; k++) {
+    hc->gdot0[k] = mat->gamma_dot_base[k];
+    hc->nexp[k] = mat->rate_sensitivity[k];
+    if (hc->nexp[k] != hc->nexp[0]) hc->uniform_n = 0;
+    nmin = std::min(nmin, hc->nexp[k]);
+  }
+  double ee = 0;
+  for (int i = 0; i < 5; i++) {
+    hc->edot[i] = edot[i];
+    ee += edot[i] * edot[i];
+  }
+  hc->imposed_max_rss = std::max(2.0, 2.0 * std::pow(ee / hc->gdot0[0], 1.0 / (double)(float)nmin));
+  if (L0)
+    for (int i = 0; i < 5; i++)
+      for (int j = 0; j < 5; j++) hc->L0[j * 5 + i] = 0.5 * (L0[j * 5 + i] + L0[i * 5 + j]);
+  else
+    for (int i = 0; i < 5; i++) hc->L0[i * 5 + i] = 1.0;
+  hc->kscale[0] = hc->kscale[1] = hc->kscale[2] = 1.0;
+  hc->nx = hc->ny = hc->nz = 2;
+  return VPFFT_OK;
+}
+}  // namespace
+
+int vpfft_solve_constrained_batch(long n, const vpfft_material* mat, const double* orientation, const double* crss,
+                                  const double L0[25], const double edot[5], const double* stress_in,
+                                  const double* lambda, const double* dtilde, double* stress_out, double* nr_error,
+                                  int* nr_steps) {
+  if (n < 1 || !mat || !orientation || !crss || !L0 || !edot || !stress_in || !lambda || !dtilde || !stress_out ||
+      !nr_error)
+    return set_error(VPFFT_E_INVALID, "bad argument");
+  int rc = vpfft::check_material(mat);
+  if (rc) return rc;
+  vpfft::StepConst hc;
+  batch_const(mat, L0, edot, &hc);
+  BatchBufs B;
+  auto* dC = (vpfft::StepConst*)B.get(sizeof hc);
+  auto* g64 = (vpfft::GreenConst<double>*)B.get(sizeof(vpfft::GreenConst<double>));
+  auto* g32 = (vpfft::GreenConst<float>*)B.get(sizeof(vpfft::GreenConst<float>));
+  auto* dR = (double*)B.get(sizeof(double) * 9 * n);
+  auto* dT = (double*)B.get(sizeof(double) * 12 * n);
+  auto* dS = (double*)B.get(sizeof(double) * 5 * n);
+  auto* dL = (double*)B.get(sizeof(double) * 5 * n);
+  auto* dD = (double*)B.get(sizeof(double) * 5 * n);
+  auto* dO = (double*)B.get(sizeof(double) * 5 * n);
+  auto* dE = (double*)B.get(sizeof(double) * n);
+  auto* dN = (int*)B.get(sizeof(int) * n);
+  if (!dC || !g64 || !g32 || !dR || !dT || !dS || !dL || !dD || !dO || !dE || !dN)
+    return set_error(VPFFT_E_NOMEM, "batch allocation failed");
+  hipError_t e = hipSuccess;
+  auto H2D = [&](void* d, const void* h, size_t b) {
+    if (e == hipSuccess) e = hipMemcpy(d, h, b, hipMemcpyHostToDevice);
+  };
+  H2D(dC, &hc, sizeof hc);
+  H2D(dR, orientation, sizeof(double) * 9 * n);
+  H2D(dT, crss, sizeof(double) * 12 * n);
+  H2D(dS, stress_in, sizeof(double) * 5 * n);
+  H2D(dL, lambda, sizeof(double) * 5 * n);
+  H2D(dD, dtilde, sizeof(double) * 5 * n);
+  H2D(dE, nr_error, sizeof(double) * n);
+  if (e == hipSuccess) e = vpfft::launch_derive_from_L0(dC, g64, g32, nullptr);
+  if (e == hipSuccess) e = vpfft::launch_constrained_batch(n, dR, dT, dC, dS, dL, dD, dO, dE, dN, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(stress_out, dO, sizeof(double) * 5 * n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(nr_error, dE, sizeof(double) * n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && nr_steps) e = hipMemcpy(nr_steps, dN, sizeof(int) * n, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return set_error(VPFFT_E_DEVICE, "constrained batch: %s", hipGetErrorString(e));
+  return VPFFT_OK;
+}
+
+int vpfft_solve_taylor_batch(long n, const vpfft_material* mat, const double* orientation, const double* crss,
+                             const double edot[5], const double* stress_in, double* stress_out) {
+  if (n < 1 || !mat || !orientation || !crss || !edot || !stress_in || !stress_out)
+    return set_error(VPFFT_E_INVALID, "bad argument");
+  int rc = vpfft::check_material(mat);
+  if (rc) return rc;
+  vpfft::StepConst hc;
+  batch_const(mat, nullptr, edot, &hc);
+  BatchBufs B;
+  auto* dC = (vpfft::StepConst*)B.get(sizeof hc);
+  auto* dR = (double*)B.get(sizeof(double) * 9 * n);
+  auto* dT = (double*)B.get(sizeof(double) * 12 * n);
+  auto* dS = (double*)B.get(sizeof(double) * 5 * n);
+  auto* dO = (double*)B.get(sizeof(double) * 5 * n);
+  if (!dC || !dR || !dT || !dS || !dO) return set_error(VPFFT_E_NOMEM, "batch allocation failed");
+  hipError_t e = hipMemcpy(dC, &hc, sizeof hc, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dR, orientation, sizeof(double) * 9 * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dT, crss, sizeof(double) * 12 * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dS, stress_in, sizeof(double) * 5 * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = vpfft::launch_taylor_batch(n, dR, dT, dC, dS, dO, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(stress_out, dO, sizeof(double) * 5 * n, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return set_error(VPFFT_E_DEVICE, "taylor batch: %s", hipGetErrorString(e));
+  return VPFFT_OK;
+}
+
+}  // extern "C"

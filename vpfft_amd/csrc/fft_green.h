@@ -1,0 +1,43 @@
+// fft_green.h -- plan/geometry structs and launchers for the FFT + Green passes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace vpfft {
+
+struct FftGeom {
+  long n;      // voxels per component (Nx*Ny*Nz)
+  long nc;     // half-spectrum bins per component (Nx*Ny*(Nz/2+1))
+  long nrows;  // Nx*Ny
+  int nx, ny, nz, nzh;
+};
+
+struct AxisPlan {
+  int n;
+  int nstages;
+  int radix[12];
+};
+
+// Per-step Green-operator constants in the transform precision.
+template <typename T>
+struct GreenConst {
+  T Kc[6][6];     // K(k)_r = sum_q Kc[r][q] mono_q(k); r over (00,11,22,12,02,01),
+                  // q over (k0k0,k1k1,k2k2,k1k2,k0k2,k0k1)
+  T kscale[3];    // 1/(N_a * voxel length_a)
+  T L0inv[25];    // column-major
+  int nx, ny, nz;
+};
+
+// Factor n into radices {8,4,2,3,5,7}; returns false for other prime factors.
+bool make_axis_plan(int n, AxisPlan* ap);
+
+// prec: 0 = FP64 storage/compute, 1 = FP32
+hipError_t launch_z_r2c(int prec, const void* f, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
+                        hipStream_t st);
+hipError_t launch_z_c2r(int prec, const void* s, void* f, const FftGeom& g, const AxisPlan& ap, const void* tw,
+                        double scale, hipStream_t st);
+hipError_t launch_y(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int sign,
+                    hipStream_t st);
+hipError_t launch_x_green(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, const void* gc,
+                          hipStream_t st);
+
+}  // namespace vpfft

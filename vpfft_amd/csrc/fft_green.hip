@@ -1,0 +1,606 @@
+// fft_green.hip -- 3-D R2C/C2R transforms of the 5-component polarization
+// field with the Green-operator convolution fused into the x pass.
+//
+// Replaces the reference's 5 forward + 5 backward in-place FFTW C2C plans and
+// its per-frequency Green loop (Src/MaterialGrid.cpp:624-761).
+//
+// Layout in HBM (component-major, like the reference's one buffer per
+// component, MaterialGrid.h:320-321):
+//   real field   f[c][x][y][z]            c = 0..4, N = Nx*Ny*Nz per component
+//   spectrum     s[c][x][y][kz]           kz = 0..Nz/2, Nc = Nx*Ny*(Nz/2+1)
+// Passes per fixed-point iteration:
+//   z_r2c : f -> s   (two real components packed in one complex FFT)
+//   y     : s in place, forward
+//   x     : s in place: forward, Green, inverse (all 5 components per block)
+//   y     : s in place, inverse
+//   z_c2r : s -> f   (scaled by 1/N)
+// Each pass stages a batch of pencils in LDS and runs a mixed-radix
+// (2,3,4,5,7,8) Stockham FFT there; global loads and stores move whole
+// contiguous runs (rows along z, runs of consecutive kz for y and x).
+//
+// The reference's special frequency planes (index N/2 and N/2+1 on every
+// even axis get L_inv instead of the Green operator, MaterialGrid.cpp:710-713)
+// are not Hermitian-symmetric.  It keeps Re(IFFT(D)) (MaterialGrid.h:354-363),
+// which equals the C2R transform of D'(k) = (D(k) + conj D(-k))/2, i.e. of the
+// operator G_eff(k) = (G(k) + G(-k))/2.  That is what the x pass applies.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "fft_green.h"
+
+namespace vpfft {
+
+template <typename T> struct C2;
+template <> struct C2<double> { using t = double2; };
+template <> struct C2<float> { using t = float2; };
+
+template <typename T> __device__ __forceinline__ typename C2<T>::t cmake(T a, T b) {
+  typename C2<T>::t r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { C r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  C r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+// multiply by (s * i), s = +-1
+template <typename C> __device__ __forceinline__ C cmul_si(C a, int s) {
+  C r;
+  r.x = s > 0 ? -a.y : a.y;
+  r.y = s > 0 ? a.x : -a.x;
+  return r;
+}
+
+// cos / sin (2 pi t / R) for the odd radices
+__device__ __constant__ static const double kCos3[3] = {1.0, -0.5, -0.5};
+__device__ __constant__ static const double kSin3[3] = {0.0, 0.86602540378443864676, -0.86602540378443864676};
+__device__ __constant__ static const double kCos5[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410,
+                                                        -0.80901699437494742410, 0.30901699437494742410};
+__device__ __constant__ static const double kSin5[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917,
+                                                        -0.58778525229247312917, -0.95105651629515357212};
+__device__ __constant__ static const double kCos7[7] = {1.0, 0.62348980185873353053, -0.22252093395631440429,
+                                                        -0.90096886790241912624, -0.90096886790241912624,
+                                                        -0.22252093395631440429, 0.62348980185873353053};
+__device__ __constant__ static const double kSin7[7] = {0.0, 0.78183148246802980871, 0.97492791218182360702,
+                                                        0.43388373911755812048, -0.43388373911755812048,
+                                                        -0.97492791218182360702, -0.78183148246802980871};
+
+template <typename T, int R>
+__device__ __forceinline__ void dft_odd(typename C2<T>::t* v, int sign) {
+  using C = typename C2<T>::t;
+  const double* cs = R == 3 ? kCos3 : (R == 5 ? kCos5 : kCos7);
+  const double* sn = R == 3 ? kSin3 : (R == 5 ? kSin5 : kSin7);
+  C out[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    T re = v[0].x, im = v[0].y;
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+      const int t = (r * k) % R;
+      const T c = (T)cs[t], s = (T)(sign * sn[t]);
+      re += v[r].x * c - v[r].y * s;
+      im += v[r].x * s + v[r].y * c;
+    }
+    out[k].x = re;
+    out[k].y = im;
+  }
+#pragma unroll
+  for (int k = 0; k < R; k++) v[k] = out[k];
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void dft(typename C2<T>::t* v, int sign) {
+  using C = typename C2<T>::t;
+  if constexpr (R == 2) {
+    C a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    C a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+    C c = cadd(v[1], v[3]), d = cmul_si(csub(v[1], v[3]), sign);
+    v[0] = cadd(a, c);
+    v[2] = csub(a, c);
+    v[1] = cadd(b, d);
+    v[3] = csub(b, d);
+  } else if constexpr (R == 8) {
+    C e[4] = {v[0], v[2], v[4], v[6]};
+    C o[4] = {v[1], v[3], v[5], v[7]};
+    dft<T, 4>(e, sign);
+    dft<T, 4>(o, sign);
+    const T h = (T)0.70710678118654752440;
+    // W8^k = exp(sign i pi k / 4)
+    C w1 = cmake<T>(h, sign * h);
+    C w3 = cmake<T>(-h, sign * h);
+    C t0 = o[0];
+    C t1 = cmul(o[1], w1);
+    C t2 = cmul_si(o[2], sign);
+    C t3 = cmul(o[3], w3);
+    v[0] = cadd(e[0], t0); v[4] = csub(e[0], t0);
+    v[1] = cadd(e[1], t1); v[5] = csub(e[1], t1);
+    v[2] = cadd(e[2], t2); v[6] = csub(e[2], t2);
+    v[3] = cadd(e[3], t3); v[7] = csub(e[3], t3);
+  } else {
+    dft_odd<T, R>(v, sign);
+  }
+}
+
+// One Stockham stage over P pencils held in LDS as buf[j * P + p].
+template <typename T, int P, int R>
+__device__ __forceinline__ void stockham_stage(const typename C2<T>::t* __restrict__ in,
+                                               typename C2<T>::t* __restrict__ out, int n, int ns,
+                                               const typename C2<T>::t* __restrict__ tw, int sign) {
+  using C = typename C2<T>::t;
+  const int nb = n / R;
+  const int total = nb * P;
+  const int twstride = n / (ns * R);
+  for (int u = threadIdx.x; u < total; u += blockDim.x) {
+    const int p = u % P;
+    const int j = u / P;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = in[(j + r * nb) * P + p];
+    const int jm = j % ns;
+    if (ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        C w = tw[r * jm * twstride];
+        if (sign > 0) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    }
+    dft<T, R>(v, sign);
+    const int idxd = (j - jm) * R + jm;
+#pragma unroll
+    for (int r = 0; r < R; r++) out[(idxd + r * ns) * P + p] = v[r];
+  }
+}
+
+// Full transform of P pencils of length n.  Data starts in buf0; returns the
+// buffer holding the result (buf0 or buf1).  Ends with a barrier.
+template <typename T, int P>
+__device__ typename C2<T>::t* lds_fft(typename C2<T>::t* buf0, typename C2<T>::t* buf1,
+                                      const AxisPlan& ap, const typename C2<T>::t* tw, int sign) {
+  using C = typename C2<T>::t;
+  C* in = buf0;
+  C* out = buf1;
+  int ns = 1;
+  for (int s = 0; s < ap.nstages; s++) {
+    const int r = ap.radix[s];
+    switch (r) {
+      case 2: stockham_stage<T, P, 2>(in, out, ap.n, ns, tw, sign); break;
+      case 3: stockham_stage<T, P, 3>(in, out, ap.n, ns, tw, sign); break;
+      case 4: stockham_stage<T, P, 4>(in, out, ap.n, ns, tw, sign); break;
+      case 5: stockham_stage<T, P, 5>(in, out, ap.n, ns, tw, sign); break;
+      case 7: stockham_stage<T, P, 7>(in, out, ap.n, ns, tw, sign); break;
+      default: stockham_stage<T, P, 8>(in, out, ap.n, ns, tw, sign); break;
+    }
+    __syncthreads();
+    ns *= r;
+    C* t = in;
+    in = out;
+    out = t;
+  }
+  return in;
+}
+
+// ---------------------------------------------------------------------------
+// z pass, real -> half spectrum.  Block = B consecutive (x,y) rows.
+// Pencil p = pair * B + b holds f[2 pair] + i f[2 pair + 1] of row b.
+// ---------------------------------------------------------------------------
+template <typename T, int B>
+__global__ void __launch_bounds__(256) k_z_r2c(const T* __restrict__ f, typename C2<T>::t* __restrict__ s,
+                                               FftGeom g, AxisPlan ap, const typename C2<T>::t* __restrict__ tw) {
+  using C = typename C2<T>::t;
+  constexpr int P = 3 * B;
+  extern __shared__ unsigned char smem_raw[];
+  C* buf0 = reinterpret_cast<C*>(smem_raw);
+  C* buf1 = buf0 + P * g.nz;
+  const long row0 = (long)blockIdx.x * B;
+  const int nrows = (int)min((long)B, g.nrows - row0);
+  const int nz = g.nz, nzh = g.nzh;
+  const int cnt = B * nz;
+  for (int pair = 0; pair < 3; pair++) {
+    const T* fa = f + (long)(2 * pair) * g.n + row0 * nz;
+    const T* fb = pair < 2 ? f + (long)(2 * pair + 1) * g.n + row0 * nz : nullptr;
+    for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
+      const int b = e / nz, j = e - b * nz;
+      T a = 0, c = 0;
+      if (b < nrows) {
+        a = fa[e];
+        if (fb) c = fb[e];
+      }
+      buf0[j * P + pair * B + b] = cmake<T>(a, c);
+    }
+  }
+  __syncthreads();
+  C* res = lds_fft<T, P>(buf0, buf1, ap, tw, -1);
+  const int cnth = B * nzh;
+  for (int pair = 0; pair < 3; pair++) {
+    C* sa = s + (long)(2 * pair) * g.nc + row0 * nzh;
+    C* sb = s + (long)(2 * pair + 1) * g.nc + row0 * nzh;
+    for (int e = threadIdx.x; e < cnth; e += blockDim.x) {
+      const int b = e / nzh, k = e - b * nzh;
+      if (b >= nrows) continue;
+      const C zk = res[k * P + pair * B + b];
+      const int kn = k == 0 ? 0 : nz - k;
+      const C zn = res[kn * P + pair * B + b];
+      // X_a = (Z_k + conj Z_{-k}) / 2 ;  X_b = -i/2 (Z_k - conj Z_{-k})
+      const T h = (T)0.5;
+      sa[e] = cmake<T>(h * (zk.x + zn.x), h * (zk.y - zn.y));
+      if (pair < 2) sb[e] = cmake<T>(h * (zk.y + zn.y), -h * (zk.x - zn.x));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// z pass, half spectrum -> real, scaled by 1/N.
+// ---------------------------------------------------------------------------
+template <typename T, int B>
+__global__ void __launch_bounds__(256) k_z_c2r(const typename C2<T>::t* __restrict__ s, T* __restrict__ f,
+                                               FftGeom g, AxisPlan ap, const typename C2<T>::t* __restrict__ tw,
+                                               T scale) {
+  using C = typename C2<T>::t;
+  constexpr int P = 3 * B;
+  extern __shared__ unsigned char smem_raw[];
+  C* buf0 = reinterpret_cast<C*>(smem_raw);
+  C* buf1 = buf0 + P * g.nz;
+  const long row0 = (long)blockIdx.x * B;
+  const int nrows = (int)min((long)B, g.nrows - row0);
+  const int nz = g.nz, nzh = g.nzh;
+  const int cnth = B * nzh;
+  const bool even = (nz & 1) == 0;
+  for (int pair = 0; pair < 3; pair++) {
+    const C* sa = s + (long)(2 * pair) * g.nc + row0 * nzh;
+    const C* sb = s + (long)(2 * pair + 1) * g.nc + row0 * nzh;
+    for (int e = threadIdx.x; e < cnth; e += blockDim.x) {
+      const int b = e / nzh, k = e - b * nzh;
+      C xa = cmake<T>(0, 0), xb = cmake<T>(0, 0);
+      if (b < nrows) {
+        xa = sa[e];
+        if (pair < 2) xb = sb[e];
+      }
+      // C2R reads only the real part of the self-conjugate bins
+      if (k == 0 || (even && 2 * k == nz)) { xa.y = 0; xb.y = 0; }
+      // Z_k = X_a + i X_b ;  Z_{-k} = conj X_a + i conj X_b
+      buf0[k * P + pair * B + b] = cmake<T>(xa.x - xb.y, xa.y + xb.x);
+      if (k > 0 && 2 * k != nz) buf0[(nz - k) * P + pair * B + b] = cmake<T>(xa.x + xb.y, xb.x - xa.y);
+    }
+  }
+  __syncthreads();
+  C* res = lds_fft<T, P>(buf0, buf1, ap, tw, +1);
+  const int cnt = B * nz;
+  for (int pair = 0; pair < 3; pair++) {
+    T* fa = f + (long)(2 * pair) * g.n + row0 * nz;
+    T* fb = f + (long)(2 * pair + 1) * g.n + row0 * nz;
+    for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
+      const int b = e / nz, j = e - b * nz;
+      if (b >= nrows) continue;
+      const C z = res[j * P + pair * B + b];
+      fa[e] = z.x * scale;
+      if (pair < 2) fb[e] = z.y * scale;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y pass, in place.  Block = (component c, x, run of B consecutive kz).
+// ---------------------------------------------------------------------------
+template <typename T, int B>
+__global__ void __launch_bounds__(256) k_y(typename C2<T>::t* __restrict__ s, FftGeom g, AxisPlan ap,
+                                           const typename C2<T>::t* __restrict__ tw, int sign) {
+  using C = typename C2<T>::t;
+  extern __shared__ unsigned char smem_raw[];
+  C* buf0 = reinterpret_cast<C*>(smem_raw);
+  C* buf1 = buf0 + B * g.ny;
+  const int runs = (g.nzh + B - 1) / B;
+  const int run = blockIdx.x % runs;
+  const long plane = blockIdx.x / runs;  // c * nx + x
+  const int kz0 = run * B;
+  const int nb = min(B, g.nzh - kz0);
+  C* base = s + plane * (long)g.ny * g.nzh + kz0;
+  const int cnt = B * g.ny;
+  for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
+    const int b = e % B, y = e / B;
+    buf0[e] = b < nb ? base[(long)y * g.nzh + b] : cmake<T>(0, 0);
+  }
+  __syncthreads();
+  C* res = lds_fft<T, B>(buf0, buf1, ap, tw, sign);
+  for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
+    const int b = e % B, y = e / B;
+    if (b < nb) base[(long)y * g.nzh + b] = res[e];
+  }
+}
+
+// Green operator applied to one frequency: phi <- G_eff(k) phi  (complex 5-vector).
+template <typename T>
+__device__ __forceinline__ void green_apply(typename C2<T>::t ph[5], int ix, int iy, int iz,
+                                            const GreenConst<T>& G) {
+  using C = typename C2<T>::t;
+  if (ix == 0 && iy == 0 && iz == 0) {  // DC -> 0, MaterialGrid.cpp:733-740
+#pragma unroll
+    for (int a = 0; a < 5; a++) ph[a] = cmake<T>(0, 0);
+    return;
+  }
+  const int nx = G.nx, ny = G.ny, nz = G.nz;
+  const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
+  const bool ex = (nx & 1) == 0, ey = (ny & 1) == 0, ez = (nz & 1) == 0;
+  const int jx = ix == 0 ? 0 : nx - ix, jy = iy == 0 ? 0 : ny - iy, jz = iz == 0 ? 0 : nz - iz;
+  const bool sp = (ex && (ix == cx || ix == cx + 1)) || (ey && (iy == cy || iy == cy + 1)) ||
+                  (ez && (iz == cz || iz == cz + 1));
+  const bool spn = (ex && (jx == cx || jx == cx + 1)) || (ey && (jy == cy || jy == cy + 1)) ||
+                   (ez && (jz == cz || jz == cz + 1));
+  C out[5];
+#pragma unroll
+  for (int a = 0; a < 5; a++) out[a] = cmake<T>(0, 0);
+  if (!(sp && spn)) {
+    // Gamma(k):  d = V N V^T phi,  V_a = B_a k,  N = K^-1 - K^-1 k k^T K^-1 / (k^T K^-1 k)
+    const T k0 = (T)(ix > cx ? ix - nx : ix) * G.kscale[0];
+    const T k1 = (T)(iy > cy ? iy - ny : iy) * G.kscale[1];
+    const T k2 = (T)(iz > cz ? iz - nz : iz) * G.kscale[2];
+    const T mono[6] = {k0 * k0, k1 * k1, k2 * k2, k1 * k2, k0 * k2, k0 * k1};
+    T K[6];
+#pragma unroll
+    for (int r = 0; r < 6; r++) {
+      T acc = 0;
+#pragma unroll
+      for (int q = 0; q < 6; q++) acc += G.Kc[r][q] * mono[q];
+      K[r] = acc;
+    }
+    // K = [[a f e],[f b d],[e d c]]
+    const T a = K[0], b = K[1], c = K[2], d = K[3], e = K[4], f = K[5];
+    const T A00 = b * c - d * d, A11 = a * c - e * e, A22 = a * b - f * f;
+    const T A12 = e * f - a * d, A02 = f * d - b * e, A01 = d * e - c * f;
+    const T det = a * A00 + f * A01 + e * A02;
+    const T p0 = A00 * k0 + A01 * k1 + A02 * k2;
+    const T p1 = A01 * k0 + A11 * k1 + A12 * k2;
+    const T p2 = A02 * k0 + A12 * k1 + A22 * k2;
+    const T sk = k0 * p0 + k1 * p1 + k2 * p2;
+    const T inv = (T)1 / (sk * det);
+    // N = (adj * sk - p p^T) / (sk det)
+    const T N00 = (A00 * sk - p0 * p0) * inv, N11 = (A11 * sk - p1 * p1) * inv,
+            N22 = (A22 * sk - p2 * p2) * inv;
+    const T N12 = (A12 * sk - p1 * p2) * inv, N02 = (A02 * sk - p0 * p2) * inv,
+            N01 = (A01 * sk - p0 * p1) * inv;
+    const T s2 = (T)0.70710678118654752440, s6 = (T)0.40824829046386301637;
+    // u = V^T phi
+    C u0, u1, u2;
+    u0.x = -s2 * k0 * ph[0].x - s6 * k0 * ph[1].x + s2 * k2 * ph[3].x + s2 * k1 * ph[4].x;
+    u0.y = -s2 * k0 * ph[0].y - s6 * k0 * ph[1].y + s2 * k2 * ph[3].y + s2 * k1 * ph[4].y;
+    u1.x = s2 * k1 * ph[0].x - s6 * k1 * ph[1].x + s2 * k2 * ph[2].x + s2 * k0 * ph[4].x;
+    u1.y = s2 * k1 * ph[0].y - s6 * k1 * ph[1].y + s2 * k2 * ph[2].y + s2 * k0 * ph[4].y;
+    u2.x = (T)2 * s6 * k2 * ph[1].x + s2 * k1 * ph[2].x + s2 * k0 * ph[3].x;
+    u2.y = (T)2 * s6 * k2 * ph[1].y + s2 * k1 * ph[2].y + s2 * k0 * ph[3].y;
+    C w0, w1, w2;
+    w0.x = N00 * u0.x + N01 * u1.x + N02 * u2.x;
+    w0.y = N00 * u0.y + N01 * u1.y + N02 * u2.y;
+    w1.x = N01 * u0.x + N11 * u1.x + N12 * u2.x;
+    w1.y = N01 * u0.y + N11 * u1.y + N12 * u2.y;
+    w2.x = N02 * u0.x + N12 * u1.x + N22 * u2.x;
+    w2.y = N02 * u0.y + N12 * u1.y + N22 * u2.y;
+    const T wt = (sp || spn) ? (T)0.5 : (T)1;
+    out[0].x = wt * s2 * (-k0 * w0.x + k1 * w1.x);
+    out[0].y = wt * s2 * (-k0 * w0.y + k1 * w1.y);
+    out[1].x = wt * s6 * (-k0 * w0.x - k1 * w1.x + (T)2 * k2 * w2.x);
+    out[1].y = wt * s6 * (-k0 * w0.y - k1 * w1.y + (T)2 * k2 * w2.y);
+    out[2].x = wt * s2 * (k2 * w1.x + k1 * w2.x);
+    out[2].y = wt * s2 * (k2 * w1.y + k1 * w2.y);
+    out[3].x = wt * s2 * (k2 * w0.x + k0 * w2.x);
+    out[3].y = wt * s2 * (k2 * w0.y + k0 * w2.y);
+    out[4].x = wt * s2 * (k1 * w0.x + k0 * w1.x);
+    out[4].y = wt * s2 * (k1 * w0.y + k0 * w1.y);
+  }
+  if (sp || spn) {
+    // special planes: d = -L_inv phi = L0^-1 phi  (MaterialGrid.cpp:628-632, 713)
+    const T wt = (sp && spn) ? (T)1 : (T)0.5;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      T re = 0, im = 0;
+#pragma unroll
+      for (int j = 0; j < 5; j++) {
+        re += G.L0inv[j * 5 + i] * ph[j].x;
+        im += G.L0inv[j * 5 + i] * ph[j].y;
+      }
+      out[i].x += wt * re;
+      out[i].y += wt * im;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 5; a++) ph[a] = out[a];
+}
+
+// ---------------------------------------------------------------------------
+// x pass with the Green operator: block = run of B consecutive columns
+// q = y * nzh + kz, all 5 components; forward FFT along x per component,
+// frequencies kept in registers (E per thread), Green applied, inverse FFT.
+// ---------------------------------------------------------------------------
+template <typename T, int B, int E>
+__global__ void __launch_bounds__(256) k_x_green(typename C2<T>::t* __restrict__ s, FftGeom g, AxisPlan ap,
+                                                 const typename C2<T>::t* __restrict__ tw,
+                                                 const GreenConst<T>* __restrict__ gc) {
+  using C = typename C2<T>::t;
+  extern __shared__ unsigned char smem_raw[];
+  C* buf0 = reinterpret_cast<C*>(smem_raw);
+  C* buf1 = buf0 + B * g.nx;
+  const long ncol = (long)g.ny * g.nzh;
+  const long q0 = (long)blockIdx.x * B;
+  const int nb = (int)min((long)B, ncol - q0);
+  const int cnt = B * g.nx;
+  C reg[5][E];
+  for (int c = 0; c < 5; c++) {
+    C* base = s + (long)c * g.nc + q0;
+    for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
+      const int b = e % B, x = e / B;
+      buf0[e] = b < nb ? base[(long)x * ncol + b] : cmake<T>(0, 0);
+    }
+    __syncthreads();
+    C* res = lds_fft<T, B>(buf0, buf1, ap, tw, -1);
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+      const int e = threadIdx.x + i * blockDim.x;
+      reg[c][i] = e < cnt ? res[e] : cmake<T>(0, 0);
+    }
+    __syncthreads();
+  }
+  const GreenConst<T> G = *gc;
+#pragma unroll
+  for (int i = 0; i < E; i++) {
+    const int e = threadIdx.x + i * blockDim.x;
+    const int b = e % B, x = e / B;
+    if (e < cnt && b < nb) {
+      const long q = q0 + b;
+      const int iy = (int)(q / g.nzh), iz = (int)(q - (long)iy * g.nzh);
+      C ph[5];
+#pragma unroll
+      for (int c = 0; c < 5; c++) ph[c] = reg[c][i];
+      green_apply<T>(ph, x, iy, iz, G);
+#pragma unroll
+      for (int c = 0; c < 5; c++) reg[c][i] = ph[c];
+    }
+  }
+  for (int c = 0; c < 5; c++) {
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+      const int e = threadIdx.x + i * blockDim.x;
+      if (e < cnt) buf0[e] = reg[c][i];
+    }
+    __syncthreads();
+    C* res = lds_fft<T, B>(buf0, buf1, ap, tw, +1);
+    C* base = s + (long)c * g.nc + q0;
+    for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
+      const int b = e % B, x = e / B;
+      if (b < nb) base[(long)x * ncol + b] = res[e];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------
+bool make_axis_plan(int n, AxisPlan* ap) {
+  ap->n = n;
+  ap->nstages = 0;
+  int m = n;
+  const int order[6] = {8, 4, 2, 3, 5, 7};
+  for (int oi = 0; oi < 6; oi++) {
+    const int r = order[oi];
+    while (m % r == 0 && m > 1) {
+      if (ap->nstages >= 12) return false;
+      ap->radix[ap->nstages++] = r;
+      m /= r;
+    }
+  }
+  return m == 1 && n > 1;
+}
+
+static int pick_rows(int nz, int budget) {
+  int b = 1;
+  while (b < 16 && 3 * (2 * b) * nz <= budget) b *= 2;
+  return b;
+}
+static int pick_run(int n, int budget) {
+  int b = 1;
+  while (b < 64 && (2 * b) * n <= budget) b *= 2;
+  return b;
+}
+
+template <typename T>
+static hipError_t launch_z_r2c_t(const T* f, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
+                                 hipStream_t st) {
+  using C = typename C2<T>::t;
+  const int B = pick_rows(g.nz, sizeof(T) == 8 ? 1536 : 3072);
+  const unsigned grid = (unsigned)((g.nrows + B - 1) / B);
+  const size_t lds = 2 * sizeof(C) * 3 * B * g.nz;
+  switch (B) {
+#define CASE(BB) case BB: hipLaunchKernelGGL((k_z_r2c<T, BB>), dim3(grid), dim3(256), lds, st, f, (C*)s, g, ap, (const C*)tw); break;
+    CASE(1) CASE(2) CASE(4) CASE(8) default: CASE(16)
+#undef CASE
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_z_c2r_t(const void* s, T* f, const FftGeom& g, const AxisPlan& ap, const void* tw,
+                                 double scale, hipStream_t st) {
+  using C = typename C2<T>::t;
+  const int B = pick_rows(g.nz, sizeof(T) == 8 ? 1536 : 3072);
+  const unsigned grid = (unsigned)((g.nrows + B - 1) / B);
+  const size_t lds = 2 * sizeof(C) * 3 * B * g.nz;
+  switch (B) {
+#define CASE(BB) case BB: hipLaunchKernelGGL((k_z_c2r<T, BB>), dim3(grid), dim3(256), lds, st, (const C*)s, f, g, ap, (const C*)tw, (T)scale); break;
+    CASE(1) CASE(2) CASE(4) CASE(8) default: CASE(16)
+#undef CASE
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_y_t(void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int sign,
+                             hipStream_t st) {
+  using C = typename C2<T>::t;
+  const int B = pick_run(g.ny, sizeof(T) == 8 ? 2048 : 4096);
+  const int runs = (g.nzh + B - 1) / B;
+  const unsigned grid = (unsigned)(5L * g.nx * runs);
+  const size_t lds = 2 * sizeof(C) * B * g.ny;
+  switch (B) {
+#define CASE(BB) case BB: hipLaunchKernelGGL((k_y<T, BB>), dim3(grid), dim3(256), lds, st, (C*)s, g, ap, (const C*)tw, sign); break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) default: CASE(64)
+#undef CASE
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_x_green_t(void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
+                                   const void* gc, hipStream_t st) {
+  using C = typename C2<T>::t;
+  // keeps 5 * E complex values per thread in registers: E <= 4
+  const int B = pick_run(g.nx, 1024);
+  const long ncol = (long)g.ny * g.nzh;
+  const unsigned grid = (unsigned)((ncol + B - 1) / B);
+  const size_t lds = 2 * sizeof(C) * B * g.nx;
+  const int cnt = B * g.nx;
+  const int E = (cnt + 255) / 256;
+  const GreenConst<T>* G = (const GreenConst<T>*)gc;
+#define LAUNCH(BB, EE) hipLaunchKernelGGL((k_x_green<T, BB, EE>), dim3(grid), dim3(256), lds, st, (C*)s, g, ap, (const C*)tw, G)
+#define EC(BB)                         \
+  case BB:                             \
+    if (E <= 1) LAUNCH(BB, 1);         \
+    else if (E <= 2) LAUNCH(BB, 2);    \
+    else if (E <= 4) LAUNCH(BB, 4);    \
+    else if (E <= 8) LAUNCH(BB, 8);    \
+    else LAUNCH(BB, 16);               \
+    break;
+  switch (B) {
+    EC(1) EC(2) EC(4) EC(8) EC(16) EC(32) default: EC(64)
+  }
+#undef EC
+#undef LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_z_r2c(int prec, const void* f, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
+                        hipStream_t st) {
+  return prec == 0 ? launch_z_r2c_t<double>((const double*)f, s, g, ap, tw, st)
+                   : launch_z_r2c_t<float>((const float*)f, s, g, ap, tw, st);
+}
+hipError_t launch_z_c2r(int prec, const void* s, void* f, const FftGeom& g, const AxisPlan& ap, const void* tw,
+                        double scale, hipStream_t st) {
+  return prec == 0 ? launch_z_c2r_t<double>(s, (double*)f, g, ap, tw, scale, st)
+                   : launch_z_c2r_t<float>(s, (float*)f, g, ap, tw, scale, st);
+}
+hipError_t launch_y(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int sign,
+                    hipStream_t st) {
+  return prec == 0 ? launch_y_t<double>(s, g, ap, tw, sign, st) : launch_y_t<float>(s, g, ap, tw, sign, st);
+}
+hipError_t launch_x_green(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, const void* gc,
+                          hipStream_t st) {
+  return prec == 0 ? launch_x_green_t<double>(s, g, ap, tw, gc, st) : launch_x_green_t<float>(s, g, ap, tw, gc, st);
+}
+
+}  // namespace vpfft

@@ -1,0 +1,66 @@
+// voxel_kernels.h -- launchers for the pointwise per-voxel kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "fft_green.h"
+
+namespace vpfft {
+
+// Fixed launch shape of every pointwise kernel: the voxel->thread assignment
+// and therefore the order of every reduction is independent of timing.
+constexpr int kPointThreads = 256;
+constexpr int kPartialStride = 16;
+
+inline int point_blocks(long n) {
+  long b = (n + kPointThreads - 1) / kPointThreads;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+// Per-iteration record written by the device (all reductions done on device).
+struct IterRecord {
+  double sums[12];      // sum sigma(5), sum edot(5), sum|sigma-lambda|, sum|edot-E-d|
+  double avg_stress[5];
+  double avg_strain_rate[5];
+  double err_stress, err_strain_rate;
+  double max_nr_error;
+  double nr_steps;      // total Newton steps taken over all voxels
+  double degenerate;    // voxels whose max resolved shear stress was < 1e-10
+  int done;
+  int iteration;
+};
+
+struct PrologueRecord {
+  double L0[25];
+  double degenerate;
+  double taylor_steps;
+};
+
+hipError_t launch_prologue(int prec, long n, const double* R, const double* crss, const StepConst* C, void* field,
+                           double* sigma, double* lambda, double* partial, hipStream_t st);
+hipError_t launch_L0_finalize(long n, int nblocks, const double* partial, StepConst* C, GreenConst<double>* g64,
+                              GreenConst<float>* g32, PrologueRecord* rec, hipStream_t st);
+// derive L0^-1 and the Green constants from C->L0 (already set)
+hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32, hipStream_t st);
+hipError_t launch_nr(int prec, long n, const void* dtil, void* phi, double* sigma, double* lambda, const double* R,
+                     const double* crss, const StepConst* C, double* partial, hipStream_t st);
+hipError_t launch_iter_finalize(long n, int nblocks, const double* partial, const StepConst* C, double tol,
+                                int max_iter, int it, IterRecord* rec, hipStream_t st);
+hipError_t launch_epilogue(long n, double* R, double* crss, double* acc, const double* sigma, const StepConst* C,
+                           hipStream_t st);
+hipError_t launch_aos_to_soa(const double* aos, double* soa, long count, int width, long soa_stride, long offset,
+                             hipStream_t st);
+hipError_t launch_soa_to_aos(const double* soa, double* aos, long count, int width, long soa_stride, long offset,
+                             hipStream_t st);
+hipError_t launch_convert(int prec_dst, void* dst, int prec_src, const void* src, long count, hipStream_t st);
+
+// Per-voxel solver batches (kernel-level access to the Solvers:: functions).
+hipError_t launch_constrained_batch(long n, const double* R, const double* crss, const StepConst* C,
+                                    const double* s_in, const double* lam, const double* dt, double* s_out,
+                                    double* err, int* steps, hipStream_t st);
+hipError_t launch_taylor_batch(long n, const double* R, const double* crss, const StepConst* C, const double* s_in,
+                               double* s_out, hipStream_t st);
+
+}  // namespace vpfft

@@ -1,0 +1,563 @@
+// voxel_kernels.hip -- pointwise per-voxel kernels of one VPFFT strain step.
+//
+//   k_prologue   : UpdateSchmidtTensors + InitializeComputation +
+//                  the per-voxel half of BuildLinearRefMediumStiffness
+//                  (MaterialGrid.cpp:107-124, 153-210, 219-250)
+//   k_L0_finalize: L0 = mean of secant-moduli inverses, NormalizeValues
+//                  (MaterialGrid.cpp:252-281), plus the per-step constants
+//                  derived from L0 (its inverse and the Green coefficients)
+//   k_nr         : the per-voxel body of the fixed-point loop
+//                  (MaterialGrid.cpp:946-1003) with its sums
+//   k_iter_final : the convergence block (MaterialGrid.cpp:1009-1068)
+//   k_epilogue   : orientation update + hardening (MaterialGrid.cpp:769-876,
+//                  1098-1157)
+//
+// Layout in HBM: structure of arrays, component-major: R[9][N], crss[12][N],
+// sigma[5][N], lambda[5][N], field[5][N] (polarization in, strain-rate
+// variation out).  One voxel per lane, grid-stride with a fixed grid so
+// every reduction has a fixed order.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "voxel_kernels.h"
+
+namespace vpfft {
+
+template <typename T> __device__ __forceinline__ double ld(const T* p) { return (double)*p; }
+
+// Reduce W values over the block in a fixed order; thread 0 gets the result.
+template <int W>
+__device__ __forceinline__ void block_sum(double (&v)[W], double (&vmax)[1], double* out) {
+  __shared__ double sh[kPointThreads / 64][W + 1];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int w = 0; w < W; w++) v[w] += __shfl_xor(v[w], off);
+    vmax[0] = fmax(vmax[0], __shfl_xor(vmax[0], off));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int w = 0; w < W; w++) sh[wave][w] = v[w];
+    sh[wave][W] = vmax[0];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < W; w++) {
+      double acc = 0.0;
+      for (int q = 0; q < kPointThreads / 64; q++) acc += sh[q][w];
+      out[w] = acc;
+    }
+    double mx = 0.0;
+    for (int q = 0; q < kPointThreads / 64; q++) mx = fmax(mx, sh[q][W]);
+    out[W] = mx;
+  }
+}
+
+__device__ __forceinline__ void load_R(const double* __restrict__ R, long n, long v, double r[9]) {
+#pragma unroll
+  for (int e = 0; e < 9; e++) r[e] = R[e * n + v];
+}
+__device__ __forceinline__ void load_inv_tau(const double* __restrict__ crss, long n, long v, double it[kSlips]) {
+#pragma unroll
+  for (int k = 0; k < kSlips; k++) it[k] = 1.0 / crss[k * n + v];
+}
+
+// Inverse of an SPD 5x5 (packed lower) by LDL^T solves against unit vectors.
+__device__ __forceinline__ void spd_inverse5(const double A[15], double Ainv[15]) {
+#pragma unroll
+  for (int c = 0; c < 5; c++) {
+    double Acopy[15];
+#pragma unroll
+    for (int e = 0; e < 15; e++) Acopy[e] = A[e];
+    double b[5] = {0, 0, 0, 0, 0}, x[5];
+    b[c] = 1.0;
+    ldlt_solve5(Acopy, b, x);
+#pragma unroll
+    for (int i = c; i < 5; i++) Ainv[sidx(i, c)] = x[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kPointThreads) k_prologue(long n, const double* __restrict__ R,
+                                                            const double* __restrict__ crss,
+                                                            const StepConst* __restrict__ Cp, T* __restrict__ field,
+                                                            double* __restrict__ sigma, double* __restrict__ lambda,
+                                                            double* __restrict__ partial) {
+  const StepConst& C = *Cp;
+  double acc[17];
+#pragma unroll
+  for (int w = 0; w < 17; w++) acc[w] = 0.0;
+  double mx[1] = {0.0};
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
+    double r[9], tau[kSlips], it[kSlips], m[kSlips][5];
+    load_R(R, n, v, r);
+#pragma unroll
+    for (int k = 0; k < kSlips; k++) {
+      tau[k] = crss[k * n + v];
+      it[k] = 1.0 / tau[k];
+    }
+    // bit-faithful: Schmid tensors, Sachs guess (Solvers.cpp:230-243), max RSS
+    // rescale (MaterialGrid.cpp:175-190), Taylor NR (Solvers.cpp:19-101)
+    faithful::schmid(r, m);
+    double guess[5];
+    faithful::sachs(m, tau, C.edot, guess);
+    const double mrss = faithful::max_rss(guess, m, tau);
+    if (mrss > 1.1) {
+#pragma unroll
+      for (int i = 0; i < 5; i++) guess[i] /= mrss;
+    }
+    if (!(mrss >= 1e-10)) acc[15] += 1.0;  // degenerate (the reference exits here)
+    double s[5];
+    faithful::taylor_nr(guess, m, tau, C, 5e-4, 1000, s);
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      sigma[i * n + v] = s[i];
+      lambda[i * n + v] = s[i];
+      field[i * n + v] = (T)s[i];
+    }
+    // secant modulus M = sum m m^T g / tau (MaterialGrid.cpp:228-243), inverted
+    double M[15];
+#pragma unroll
+    for (int e = 0; e < 15; e++) M[e] = 0.0;
+#pragma unroll
+    for (int k = 0; k < kSlips; k++) {
+      const double rs = dot5(m[k], s) * it[k];
+      const double c = gamma_nm1_taylor(C, k, fabs(rs)) * it[k];
+#pragma unroll
+      for (int i = 0; i < 5; i++)
+#pragma unroll
+        for (int j = 0; j <= i; j++) M[sidx(i, j)] += c * m[k][i] * m[k][j];
+    }
+    double Mi[15];
+    spd_inverse5(M, Mi);
+#pragma unroll
+    for (int e = 0; e < 15; e++) acc[e] += Mi[e];
+  }
+  block_sum<17>(acc, mx, partial + (long)blockIdx.x * 18);
+}
+
+// L0 = (1/N) sum of the per-voxel inverses; derived constants.
+__device__ void derive_constants(StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32) {
+  // L0^-1
+  double A[15], Ai[15];
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j <= i; j++) A[sidx(i, j)] = C->L0[j * 5 + i];
+  spd_inverse5(A, Ai);
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) C->L0inv[j * 5 + i] = Ai[sidx(i, j)];
+  // L_ijkl = sum_ab L0(a,b) B_a,ij B_b,kl ; K_ik = sum_jl L_ijkl k_j k_l
+  const double s2 = 0.70710678118654752440, s6 = 0.40824829046386301637;
+  double Bm[5][3][3];
+  for (int a = 0; a < 5; a++)
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) Bm[a][i][j] = 0.0;
+  Bm[0][0][0] = -s2; Bm[0][1][1] = s2;
+  Bm[1][0][0] = -s6; Bm[1][1][1] = -s6; Bm[1][2][2] = 2.0 * s6;
+  Bm[2][1][2] = s2; Bm[2][2][1] = s2;
+  Bm[3][0][2] = s2; Bm[3][2][0] = s2;
+  Bm[4][0][1] = s2; Bm[4][1][0] = s2;
+  const int ri[6] = {0, 1, 2, 1, 0, 0}, rk[6] = {0, 1, 2, 2, 2, 1};
+  const int qj[6] = {0, 1, 2, 1, 0, 0}, ql[6] = {0, 1, 2, 2, 2, 1};
+  for (int r = 0; r < 6; r++)
+    for (int q = 0; q < 6; q++) {
+      const int i = ri[r], k = rk[r], j = qj[q], l = ql[q];
+      double c = 0.0;
+      for (int a = 0; a < 5; a++)
+        for (int b = 0; b < 5; b++) {
+          const double L = C->L0[b * 5 + a];
+          c += L * Bm[a][i][j] * Bm[b][k][l];
+          if (j != l) c += L * Bm[a][i][l] * Bm[b][k][j];
+        }
+      g64->Kc[r][q] = c;
+      g32->Kc[r][q] = (float)c;
+    }
+  for (int e = 0; e < 25; e++) {
+    g64->L0inv[e] = C->L0inv[e];
+    g32->L0inv[e] = (float)C->L0inv[e];
+  }
+  for (int a = 0; a < 3; a++) {
+    g64->kscale[a] = C->kscale[a];
+    g32->kscale[a] = (float)C->kscale[a];
+  }
+  g64->nx = g32->nx = C->nx;
+  g64->ny = g32->ny = C->ny;
+  g64->nz = g32->nz = C->nz;
+}
+
+__global__ void __launch_bounds__(256) k_L0_finalize(long n, int nblocks, const double* __restrict__ partial,
+                                                     StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32,
+                                                     PrologueRecord* rec) {
+  __shared__ double sh[256][18];
+  double acc[18];
+  for (int w = 0; w < 18; w++) acc[w] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+    for (int w = 0; w < 18; w++) acc[w] += partial[(long)b * 18 + w];
+  for (int w = 0; w < 18; w++) sh[threadIdx.x][w] = acc[w];
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride)
+      for (int w = 0; w < 18; w++) sh[threadIdx.x][w] += sh[threadIdx.x + stride][w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double inv_n = 1.0 / (double)n;
+    for (int i = 0; i < 5; i++)
+      for (int j = 0; j <= i; j++) {
+        double v = sh[0][sidx(i, j)] * inv_n;
+        if (fabs(v) < 1e-10) v = 0.0;  // NormalizeValues, LinearAlgebra.h:605-613
+        C->L0[j * 5 + i] = v;
+        C->L0[i * 5 + j] = v;
+      }
+    derive_constants(C, g64, g32);
+    for (int e = 0; e < 25; e++) rec->L0[e] = C->L0[e];
+    rec->degenerate = sh[0][15];
+    rec->taylor_steps = 0.0;
+  }
+}
+
+__global__ void k_derive(StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) derive_constants(C, g64, g32);
+}
+
+// ---------------------------------------------------------------------------
+// K4: per-voxel body of the fixed-point loop.
+// partial per block: [0..4] sum sigma, [5..9] sum edot, [10] sum|sigma-lambda|,
+// [11] sum|edot-E-d|, [12] NR steps, [13] degenerate, [14] max NR error
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kPointThreads) k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
+                                                      double* __restrict__ sigma,
+                                                      double* __restrict__ lambda, const double* __restrict__ R,
+                                                      const double* __restrict__ crss,
+                                                      const StepConst* __restrict__ Cp,
+                                                      double* __restrict__ partial) {
+  const StepConst& C = *Cp;
+  double acc[14];
+#pragma unroll
+  for (int w = 0; w < 14; w++) acc[w] = 0.0;
+  double mx[1] = {0.0};
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
+    double r[9], it[kSlips], m[kSlips][5];
+    double d[5], s[5], lam[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      d[i] = ld(dtil + i * n + v);
+      s[i] = sigma[i * n + v];
+      lam[i] = lambda[i * n + v];
+    }
+    load_R(R, n, v, r);
+    load_inv_tau(crss, n, v, it);
+    schmid_from_R(r, m);
+    const double mrss = max_rss(s, m, it);
+    if (!(mrss >= 1e-10)) acc[13] += 1.0;
+#pragma unroll
+    for (int i = 0; i < 5; i++) s[i] /= mrss;
+    double q[5], Ll[5];
+    matvec5(C.L0inv, lam, Ll);
+#pragma unroll
+    for (int i = 0; i < 5; i++) q[i] = Ll[i] + C.edot[i] + d[i];
+    double err = 0.0;
+    int steps = 0;
+    constrained_nr(s, m, it, C, q, 5e-4, 1e9, 100, s, &err, &steps);
+    double e[5];
+    apply_constitutive(s, m, it, C, e);
+    double Ld[5], t[5], Lt[5];
+    matvec5(C.L0, d, Ld);
+#pragma unroll
+    for (int i = 0; i < 5; i++) t[i] = C.edot[i] + d[i] - e[i];
+    matvec5(C.L0, t, Lt);
+    double se[5], ee[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      lam[i] += Lt[i];
+      phi[i * n + v] = (T)(s[i] - Ld[i]);
+      sigma[i * n + v] = s[i];
+      lambda[i * n + v] = lam[i];
+      se[i] = s[i] - lam[i];
+      ee[i] = e[i] - C.edot[i] - d[i];
+      acc[i] += s[i];
+      acc[5 + i] += e[i];
+    }
+    acc[10] += sqrt(dot5(se, se));
+    acc[11] += sqrt(dot5(ee, ee));
+    acc[12] += (double)steps;
+    mx[0] = fmax(mx[0], err);
+  }
+  block_sum<14>(acc, mx, partial + (long)blockIdx.x * kPartialStride);
+}
+
+__global__ void __launch_bounds__(256) k_iter_finalize(long n, int nblocks, const double* __restrict__ partial,
+                                                       const StepConst* __restrict__ C, double tol, int max_iter,
+                                                       int it, IterRecord* rec) {
+  __shared__ double sh[256][15];
+  double acc[15];
+  for (int w = 0; w < 15; w++) acc[w] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    for (int w = 0; w < 14; w++) acc[w] += partial[(long)b * kPartialStride + w];
+    acc[14] = fmax(acc[14], partial[(long)b * kPartialStride + 14]);
+  }
+  for (int w = 0; w < 15; w++) sh[threadIdx.x][w] = acc[w];
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) {
+      for (int w = 0; w < 14; w++) sh[threadIdx.x][w] += sh[threadIdx.x + stride][w];
+      sh[threadIdx.x][14] = fmax(sh[threadIdx.x][14], sh[threadIdx.x + stride][14]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    IterRecord& R = rec[it];
+    // MaterialGrid.cpp:1036-1053
+    const double nd = (double)n;
+    for (int w = 0; w < 12; w++) R.sums[w] = sh[0][w];
+    double es = sh[0][10] / nd, ee = sh[0][11] / nd;
+    double avg[5], avge[5];
+    for (int i = 0; i < 5; i++) {
+      avg[i] = sh[0][i] / nd;
+      avge[i] = sh[0][5 + i] / nd;
+      R.avg_stress[i] = avg[i];
+      R.avg_strain_rate[i] = avge[i];
+    }
+    es /= sqrt(dot5(avg, avg));
+    ee /= sqrt(dot5(C->edot, C->edot));
+    R.err_stress = es;
+    R.err_strain_rate = ee;
+    R.nr_steps = sh[0][12];
+    R.degenerate = sh[0][13];
+    R.max_nr_error = sh[0][14];
+    R.iteration = it + 1;
+    // loop condition, MaterialGrid.cpp:1066-1068
+    R.done = !((it + 1) < max_iter && (es > tol || ee > tol));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Epilogue: R += dt * sum_k (R a0_k R^T) gdot_k sgn(r_k) with the old Schmid
+// tensors, then Voce hardening with the new ones.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kPointThreads) k_epilogue(long n, double* __restrict__ R, double* __restrict__ crss,
+                                                            double* __restrict__ acc_shear,
+                                                            const double* __restrict__ sigma,
+                                                            const StepConst* __restrict__ Cp) {
+  const StepConst& C = *Cp;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
+    double r[9], tau[kSlips], it[kSlips], m[kSlips][5], s[5];
+    load_R(R, n, v, r);
+#pragma unroll
+    for (int k = 0; k < kSlips; k++) { tau[k] = crss[k * n + v]; it[k] = 1.0 / tau[k]; }
+#pragma unroll
+    for (int i = 0; i < 5; i++) s[i] = sigma[i * n + v];
+    schmid_from_R(r, m);
+    double w01 = 0, w02 = 0, w12 = 0;
+#pragma unroll
+    for (int k = 0; k < kSlips; k++) {
+      const double rs = dot5(m[k], s) * it[k];
+      const double g = gamma_n(C, k, fabs(rs)) * sgn(rs);
+      double W[3];
+      anti_from_R(r, k, W);
+      w01 += W[0] * g;
+      w02 += W[1] * g;
+      w12 += W[2] * g;
+    }
+    const double dt = C.dt;
+    const double Wm[9] = {0.0, w01, w02, -w01, 0.0, w12, -w02, -w12, 0.0};
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+      r[e] += Wm[e] * dt;
+      R[e * n + v] = r[e];
+    }
+    // hardening with the updated Schmid tensors, Solvers.cpp:314-373
+    schmid_from_R(r, m);
+    double g[kSlips], local = 0.0;
+#pragma unroll
+    for (int k = 0; k < kSlips; k++) {
+      const double rs = dot5(m[k], s) * it[k];
+      g[k] = gamma_n(C, k, fabs(rs));
+      local += fabs(g[k]) * dt;
+    }
+    const double A = acc_shear[v];
+#pragma unroll
+    for (int i = 0; i < kSlips; i++) {
+      double dtau = 0.0;
+#pragma unroll
+      for (int j = 0; j < kSlips; j++) dtau += C.hmat[i * kSlips + j] * fabs(g[j]) * dt;
+      const double tiny = 1e-4 * C.tau0[i];
+      double voce = 0.0;
+      if (fabs(C.theta0[i]) > tiny) {
+        voce = C.theta1[i] * local;
+        if (fabs(C.tau1[i]) > tiny) {
+          const double fr = fabs(C.theta0[i] / C.tau1[i]);
+          const double e_ini = exp(-A * fr);
+          const double e_del = exp(-local * fr);
+          voce = voce - (fr * C.tau1[i] * C.theta1[i]) / fr * e_ini * (e_del - 1.0) -
+                 C.theta1[i] / fr * e_ini * (e_del * ((A + local) * fr + 1.0) - (A * fr + 1.0));
+        }
+      }
+      crss[i * n + v] = tau[i] + dtau * voce / local;
+    }
+    acc_shear[v] = A + local;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversions (host AoS <-> device SoA), chunked through staging.
+// ---------------------------------------------------------------------------
+__global__ void k_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, long count, int width,
+                             long stride, long offset) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < count * width; t += (long)gridDim.x * blockDim.x) {
+    const long v = t / width;
+    const int c = (int)(t - v * width);
+    soa[c * stride + offset + v] = aos[t];
+  }
+}
+__global__ void k_soa_to_aos(const double* __restrict__ soa, double* __restrict__ aos, long count, int width,
+                             long stride, long offset) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < count * width; t += (long)gridDim.x * blockDim.x) {
+    const long v = t / width;
+    const int c = (int)(t - v * width);
+    aos[t] = soa[c * stride + offset + v];
+  }
+}
+template <typename D, typename S>
+__global__ void k_convert(D* __restrict__ dst, const S* __restrict__ src, long count) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long)gridDim.x * blockDim.x)
+    dst[t] = (D)src[t];
+}
+
+// ---------------------------------------------------------------------------
+// Per-voxel solver batches, AoS inputs (voxel-contiguous), for kernel-level
+// parity and the Solvers:: facade.
+// ---------------------------------------------------------------------------
+__global__ void k_constrained_batch(long n, const double* __restrict__ R, const double* __restrict__ crss,
+                                    const StepConst* __restrict__ Cp, const double* __restrict__ s_in,
+                                    const double* __restrict__ lam, const double* __restrict__ dtl,
+                                    double* __restrict__ s_out, double* __restrict__ err, int* __restrict__ steps) {
+  const StepConst& C = *Cp;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
+    double r[9], it[kSlips], m[kSlips][5], s[5], l[5], d[5];
+    for (int e = 0; e < 9; e++) r[e] = R[9 * v + e];
+    for (int k = 0; k < kSlips; k++) it[k] = 1.0 / crss[12 * v + k];
+    for (int i = 0; i < 5; i++) { s[i] = s_in[5 * v + i]; l[i] = lam[5 * v + i]; d[i] = dtl[5 * v + i]; }
+    schmid_from_R(r, m);
+    double Ll[5], q[5];
+    matvec5(C.L0inv, l, Ll);
+    for (int i = 0; i < 5; i++) q[i] = Ll[i] + C.edot[i] + d[i];
+    double e = err[v];
+    int st = 0;
+    constrained_nr(s, m, it, C, q, 5e-4, 1e9, 100, s, &e, &st);
+    for (int i = 0; i < 5; i++) s_out[5 * v + i] = s[i];
+    err[v] = e;
+    steps[v] = st;
+  }
+}
+
+__global__ void k_taylor_batch(long n, const double* __restrict__ R, const double* __restrict__ crss,
+                               const StepConst* __restrict__ Cp, const double* __restrict__ s_in,
+                               double* __restrict__ s_out) {
+  const StepConst& C = *Cp;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
+    double r[9], tau[kSlips], m[kSlips][5], s[5];
+    for (int e = 0; e < 9; e++) r[e] = R[9 * v + e];
+    for (int k = 0; k < kSlips; k++) tau[k] = crss[12 * v + k];
+    for (int i = 0; i < 5; i++) s[i] = s_in[5 * v + i];
+    faithful::schmid(r, m);
+    faithful::taylor_nr(s, m, tau, C, 5e-4, 1000, s);
+    for (int i = 0; i < 5; i++) s_out[5 * v + i] = s[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_prologue(int prec, long n, const double* R, const double* crss, const StepConst* C, void* field,
+                           double* sigma, double* lambda, double* partial, hipStream_t st) {
+  const int nb = point_blocks(n);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_prologue<double>, dim3(nb), dim3(kPointThreads), 0, st, n, R, crss, C, (double*)field,
+                       sigma, lambda, partial);
+  else
+    hipLaunchKernelGGL(k_prologue<float>, dim3(nb), dim3(kPointThreads), 0, st, n, R, crss, C, (float*)field,
+                       sigma, lambda, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_L0_finalize(long n, int nblocks, const double* partial, StepConst* C, GreenConst<double>* g64,
+                              GreenConst<float>* g32, PrologueRecord* rec, hipStream_t st) {
+  hipLaunchKernelGGL(k_L0_finalize, dim3(1), dim3(256), 0, st, n, nblocks, partial, C, g64, g32, rec);
+  return hipGetLastError();
+}
+
+hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32, hipStream_t st) {
+  hipLaunchKernelGGL(k_derive, dim3(1), dim3(64), 0, st, C, g64, g32);
+  return hipGetLastError();
+}
+
+hipError_t launch_nr(int prec, long n, const void* dtil, void* phi, double* sigma, double* lambda, const double* R,
+                     const double* crss, const StepConst* C, double* partial, hipStream_t st) {
+  const int nb = point_blocks(n);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_nr<double>, dim3(nb), dim3(kPointThreads), 0, st, n, (const double*)dtil, (double*)phi,
+                       sigma, lambda, R, crss, C, partial);
+  else
+    hipLaunchKernelGGL(k_nr<float>, dim3(nb), dim3(kPointThreads), 0, st, n, (const float*)dtil, (float*)phi,
+                       sigma, lambda, R, crss, C, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_iter_finalize(long n, int nblocks, const double* partial, const StepConst* C, double tol,
+                                int max_iter, int it, IterRecord* rec, hipStream_t st) {
+  hipLaunchKernelGGL(k_iter_finalize, dim3(1), dim3(256), 0, st, n, nblocks, partial, C, tol, max_iter, it, rec);
+  return hipGetLastError();
+}
+
+hipError_t launch_epilogue(long n, double* R, double* crss, double* acc, const double* sigma, const StepConst* C,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_epilogue, dim3(point_blocks(n)), dim3(kPointThreads), 0, st, n, R, crss, acc, sigma, C);
+  return hipGetLastError();
+}
+
+static int cvt_blocks(long work) {
+  long b = (work + 255) / 256;
+  if (b > 8192) b = 8192;
+  return (int)(b < 1 ? 1 : b);
+}
+
+hipError_t launch_aos_to_soa(const double* aos, double* soa, long count, int width, long stride, long offset,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(cvt_blocks(count * width)), dim3(256), 0, st, aos, soa, count, width, stride,
+                     offset);
+  return hipGetLastError();
+}
+hipError_t launch_soa_to_aos(const double* soa, double* aos, long count, int width, long stride, long offset,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(cvt_blocks(count * width)), dim3(256), 0, st, soa, aos, count, width, stride,
+                     offset);
+  return hipGetLastError();
+}
+hipError_t launch_convert(int prec_dst, void* dst, int prec_src, const void* src, long count, hipStream_t st) {
+  const dim3 grid(cvt_blocks(count));
+  if (prec_dst == 0 && prec_src == 1)
+    hipLaunchKernelGGL((k_convert<double, float>), grid, dim3(256), 0, st, (double*)dst, (const float*)src, count);
+  else if (prec_dst == 1 && prec_src == 0)
+    hipLaunchKernelGGL((k_convert<float, double>), grid, dim3(256), 0, st, (float*)dst, (const double*)src, count);
+  else if (prec_dst == 0)
+    hipLaunchKernelGGL((k_convert<double, double>), grid, dim3(256), 0, st, (double*)dst, (const double*)src, count);
+  else
+    hipLaunchKernelGGL((k_convert<float, float>), grid, dim3(256), 0, st, (float*)dst, (const float*)src, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_constrained_batch(long n, const double* R, const double* crss, const StepConst* C,
+                                    const double* s_in, const double* lam, const double* dt, double* s_out,
+                                    double* err, int* steps, hipStream_t st) {
+  hipLaunchKernelGGL(k_constrained_batch, dim3(point_blocks(n)), dim3(kPointThreads), 0, st, n, R, crss, C, s_in,
+                     lam, dt, s_out, err, steps);
+  return hipGetLastError();
+}
+hipError_t launch_taylor_batch(long n, const double* R, const double* crss, const StepConst* C, const double* s_in,
+                               double* s_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_taylor_batch, dim3(point_blocks(n)), dim3(kPointThreads), 0, st, n, R, crss, C, s_in, s_out);
+  return hipGetLastError();
+}
+
+}  // namespace vpfft
