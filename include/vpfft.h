@@ -87,6 +87,7 @@ typedef struct {
   double nr_steps_last;                        /* Newton steps summed over voxels, last iteration */
   double nr_steps_total;                       /* ... summed over all iterations */
   double ms_prologue, ms_loop, ms_epilogue;    /* device time (HIP events) */
+  double max_nr_steps;                         /* largest per-voxel Newton step count, last iteration */
 } vpfft_step_info;
 
 /* ---- grid lifetime: MaterialGrid(int,int,int) + InitializeMemory (MaterialGrid.cpp:312-394) */

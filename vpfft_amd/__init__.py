@@ -83,6 +83,7 @@ class vpfft_step_info(C.Structure):
         ("ms_prologue", C.c_double),
         ("ms_loop", C.c_double),
         ("ms_epilogue", C.c_double),
+        ("max_nr_steps", C.c_double),
     ]
 
     def as_dict(self) -> dict:

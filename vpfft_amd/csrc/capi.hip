@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -92,6 +93,7 @@ struct Grid {
   long staging_elems = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool uploaded = false;
+  int nr_waves = 2;  // occupancy variant of the Newton kernel (VPFFT_NR_WAVES=1|2)
   int phi_prec = 0, dtil_prec = 0;  // precision the field buffers currently hold
   // per-kernel timing with HIP events on the grid's stream
   bool profile = false;
@@ -334,6 +336,7 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
   g->nc = (long)d->nx * d->ny * g->nzh;
   g->prec = d->precision;
   g->mat = *mat;
+  if (const char* w = std::getenv("VPFFT_NR_WAVES")) g->nr_waves = std::atoi(w) >= 2 ? 2 : 1;
   const int dims[3] = {g->nx, g->ny, g->nz};
   for (int a = 0; a < 3; a++)
     if (!make_axis_plan(dims[a], &g->plan[a])) {
@@ -444,7 +447,8 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
   for (;;) {
     if ((rc = run_fft_green(g, prec))) return rc;
     kbegin(g, K_NR);
-    HIP_TRY(launch_nr(prec, n, g->dtil, g->phi, g->sigma, g->lambda, g->R, g->crss, g->dC, g->partial, g->st));
+    HIP_TRY(launch_nr(prec, g->nr_waves, n, g->dtil, g->phi, g->sigma, g->lambda, g->R, g->crss, g->dC, g->partial,
+                      g->st));
     kend(g, K_NR);
     g->phi_prec = prec;
     kbegin(g, K_ITER_FINALIZE);
@@ -496,6 +500,7 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
     }
     for (int e = 0; e < 25; e++) info->L0[e] = g->hprol->L0[e];
     info->max_nr_error = last.max_nr_error;
+    info->max_nr_steps = last.max_nr_steps;
     info->nr_steps_last = last.nr_steps;
     info->nr_steps_total = nr_total;
     float ms = 0;

@@ -161,6 +161,41 @@ __device__ __forceinline__ void ldlt_solve5(double A[15], const double b[5], dou
   }
 }
 
+// In-place variant: A's packed lower triangle is overwritten by L (unit
+// diagonal implied), the solution replaces b.  Same arithmetic as ldlt_solve5.
+__device__ __forceinline__ void ldlt_solve5_inplace(double A[15], double b[5]) {
+  double dinv[5];
+#pragma unroll
+  for (int j = 0; j < 5; j++) {
+    double d = A[sidx(j, j)];
+#pragma unroll
+    for (int k = 0; k < j; k++) d -= A[sidx(j, k)] * A[sidx(j, k)] * A[sidx(k, k)];
+    const double scale = fabs(A[sidx(j, j)]);
+    const bool ok = d > 1e-300 && d > 1e-14 * scale;
+    const double dj = ok ? d : 0.0;
+    dinv[j] = ok ? 1.0 / d : 0.0;
+#pragma unroll
+    for (int i = j + 1; i < 5; i++) {
+      double t = A[sidx(i, j)];
+#pragma unroll
+      for (int k = 0; k < j; k++) t -= A[sidx(i, k)] * A[sidx(j, k)] * A[sidx(k, k)];
+      A[sidx(i, j)] = t * dinv[j];
+    }
+    A[sidx(j, j)] = dj;  // D stored on the diagonal
+  }
+#pragma unroll
+  for (int i = 0; i < 5; i++)
+#pragma unroll
+    for (int k = 0; k < i; k++) b[i] -= A[sidx(i, k)] * b[k];
+#pragma unroll
+  for (int i = 4; i >= 0; i--) {
+    double t = b[i] * dinv[i];
+#pragma unroll
+    for (int k = i + 1; k < 5; k++) t -= A[sidx(k, i)] * b[k];
+    b[i] = t;
+  }
+}
+
 __device__ __forceinline__ void matvec5(const double* __restrict__ Lcm, const double v[5],
                                         double out[5]) {
   // SMatrix5x5 * EigenRep, LinearAlgebra.cpp:1120-1127 (column-major L)
@@ -273,39 +308,34 @@ __device__ __forceinline__ void taylor_nr(const double init[5], const double m[k
 // Solvers::SolveConstrainedConstitutiveEquations, Solvers.cpp:111-222.
 // q = L0^-1 lambda + edot + dtilde (constant over the solve).  *err_out is
 // left untouched when the budget runs out inside the back-track (:166-170).
-__device__ __forceinline__ void constrained_nr(const double init[5], const double m[kSlips][5],
+// Register-lean form: resolved shear stresses are recomputed where needed
+// rather than kept (12 doubles), the iterate doubles as the returned value,
+// and the 5x5 system is factored in place.
+__device__ __forceinline__ void constrained_nr(double cur[5], const double m[kSlips][5],
                                                const double inv_tau[kSlips], const StepConst& C,
                                                const double q[5], double eps, double max_rss,
-                                               int max_it, double out[5], double* err_out,
-                                               int* steps_out) {
-  double cur[5], saved[5];
-  double nw[5] = {-1.0, -3.0, -7.0, -9.0, -11.0};
+                                               int max_it, double* err_out, int* steps_out) {
+  double saved[5];
   double rel = 1e5;
 #pragma unroll
-  for (int i = 0; i < 5; i++) { cur[i] = init[i]; saved[i] = init[i]; }
+  for (int i = 0; i < 5; i++) saved[i] = cur[i];
   int rem = max_it;
   int steps = 0;
   while (rem > 0) {
-    double rss[kSlips];
     bool ok;
     do {
-      ok = true;
+      double mx = 0.0;
 #pragma unroll
-      for (int k = 0; k < kSlips; k++) {
-        const double r = dot5(m[k], cur) * inv_tau[k];
-        ok = ok && !(fabs(r) > max_rss);
-        rss[k] = r;
-      }
+      for (int k = 0; k < kSlips; k++) mx = fmax(mx, fabs(dot5(m[k], cur) * inv_tau[k]));
+      ok = !(mx > max_rss);
       if (!ok) {
 #pragma unroll
         for (int i = 0; i < 5; i++) cur[i] = saved[i] + (cur[i] - saved[i]) * 0.2;
       }
       rem--;
       if (rem < 0) {
-#pragma unroll
-        for (int i = 0; i < 5; i++) out[i] = cur[i];
         *steps_out = steps;
-        return;
+        return;  // cur is the returned state (Solvers.cpp:169)
       }
     } while (!ok);
     steps++;
@@ -317,39 +347,44 @@ __device__ __forceinline__ void constrained_nr(const double init[5], const doubl
       for (int j = 0; j <= i; j++, e++) A[e] = C.L0inv[j * 5 + i];
 #pragma unroll
     for (int k = 0; k < kSlips; k++) {
-      const double g = gamma_nm1(C, k, fabs(rss[k]));
-      const double gr = g * rss[k];
+      const double r = dot5(m[k], cur) * inv_tau[k];
+      const double g = gamma_nm1(C, k, fabs(r));
+      const double gr = g * r;
       const double ck = (double)nexp_of(C, k) * g * inv_tau[k];
 #pragma unroll
       for (int i = 0; i < 5; i++) {
         S[i] += m[k][i] * gr;
+        const double cm = ck * m[k][i];
 #pragma unroll
-        for (int j = 0; j <= i; j++) A[sidx(i, j)] += ck * m[k][i] * m[k][j];
+        for (int j = 0; j <= i; j++) A[sidx(i, j)] += cm * m[k][j];
       }
     }
     // L0^-1 Res = L0^-1 cur + S - q  ;  delta = -(L0^-1 + J)^-1 L0^-1 Res
-    double Lc[5], rhs[5], delta[5];
-    matvec5(C.L0inv, cur, Lc);
+    double rhs[5];
 #pragma unroll
-    for (int i = 0; i < 5; i++) rhs[i] = q[i] - Lc[i] - S[i];
-    ldlt_solve5(A, rhs, delta);
-    double diff[5], avg[5];
+    for (int i = 0; i < 5; i++) {
+      double Lc = 0.0;
+#pragma unroll
+      for (int j = 0; j < 5; j++) Lc += C.L0inv[j * 5 + i] * cur[j];
+      rhs[i] = q[i] - Lc - S[i];
+    }
+    ldlt_solve5_inplace(A, rhs);
+    double dn = 0.0, an = 0.0;
 #pragma unroll
     for (int i = 0; i < 5; i++) {
       saved[i] = cur[i];
-      nw[i] = cur[i] + delta[i];
-      diff[i] = nw[i] - saved[i];
-      avg[i] = (nw[i] + saved[i]) * 0.5;
+      const double nw = cur[i] + rhs[i];
+      const double diff = nw - saved[i];
+      const double avg = (nw + saved[i]) * 0.5;
+      dn += diff * diff;
+      an += avg * avg;
+      cur[i] = nw;
     }
-    rel = norm5(diff) / norm5(avg);
-#pragma unroll
-    for (int i = 0; i < 5; i++) cur[i] = nw[i];
+    rel = sqrt(dn) / sqrt(an);
     if (rel < eps) break;
   }
   *err_out = rel;
   *steps_out = steps;
-#pragma unroll
-  for (int i = 0; i < 5; i++) out[i] = nw[i];
 }
 
 // Solvers::ApplyConstitutiveEquations, Solvers.cpp:255-274.

@@ -28,6 +28,7 @@ struct IterRecord {
   double max_nr_error;
   double nr_steps;      // total Newton steps taken over all voxels
   double degenerate;    // voxels whose max resolved shear stress was < 1e-10
+  double max_nr_steps;  // largest Newton step count of any voxel
   int done;
   int iteration;
 };
@@ -44,8 +45,9 @@ hipError_t launch_L0_finalize(long n, int nblocks, const double* partial, StepCo
                               GreenConst<float>* g32, PrologueRecord* rec, hipStream_t st);
 // derive L0^-1 and the Green constants from C->L0 (already set)
 hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32, hipStream_t st);
-hipError_t launch_nr(int prec, long n, const void* dtil, void* phi, double* sigma, double* lambda, const double* R,
-                     const double* crss, const StepConst* C, double* partial, hipStream_t st);
+// waves: 1 or 2 waves per SIMD (register budget 512 or 256 per lane)
+hipError_t launch_nr(int prec, int waves, long n, const void* dtil, void* phi, double* sigma, double* lambda,
+                     const double* R, const double* crss, const StepConst* C, double* partial, hipStream_t st);
 hipError_t launch_iter_finalize(long n, int nblocks, const double* partial, const StepConst* C, double tol,
                                 int max_iter, int it, IterRecord* rec, hipStream_t st);
 hipError_t launch_epilogue(long n, double* R, double* crss, double* acc, const double* sigma, const StepConst* C,

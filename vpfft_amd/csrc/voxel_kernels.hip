@@ -26,20 +26,23 @@ namespace vpfft {
 template <typename T> __device__ __forceinline__ double ld(const T* p) { return (double)*p; }
 
 // Reduce W values over the block in a fixed order; thread 0 gets the result.
-template <int W>
-__device__ __forceinline__ void block_sum(double (&v)[W], double (&vmax)[1], double* out) {
-  __shared__ double sh[kPointThreads / 64][W + 1];
+// W sums then M maxima go to out[0..W+M).
+template <int W, int M = 1>
+__device__ __forceinline__ void block_sum(double (&v)[W], double (&vmax)[M], double* out) {
+  __shared__ double sh[kPointThreads / 64][W + M];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
     for (int w = 0; w < W; w++) v[w] += __shfl_xor(v[w], off);
-    vmax[0] = fmax(vmax[0], __shfl_xor(vmax[0], off));
+#pragma unroll
+    for (int w = 0; w < M; w++) vmax[w] = fmax(vmax[w], __shfl_xor(vmax[w], off));
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) {
 #pragma unroll
     for (int w = 0; w < W; w++) sh[wave][w] = v[w];
-    sh[wave][W] = vmax[0];
+#pragma unroll
+    for (int w = 0; w < M; w++) sh[wave][W + w] = vmax[w];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -48,9 +51,11 @@ __device__ __forceinline__ void block_sum(double (&v)[W], double (&vmax)[1], dou
       for (int q = 0; q < kPointThreads / 64; q++) acc += sh[q][w];
       out[w] = acc;
     }
-    double mx = 0.0;
-    for (int q = 0; q < kPointThreads / 64; q++) mx = fmax(mx, sh[q][W]);
-    out[W] = mx;
+    for (int w = 0; w < M; w++) {
+      double mx = 0.0;
+      for (int q = 0; q < kPointThreads / 64; q++) mx = fmax(mx, sh[q][W + w]);
+      out[W + w] = mx;
+    }
   }
 }
 
@@ -226,8 +231,9 @@ __global__ void k_derive(StepConst* C, GreenConst<double>* g64, GreenConst<float
 // partial per block: [0..4] sum sigma, [5..9] sum edot, [10] sum|sigma-lambda|,
 // [11] sum|edot-E-d|, [12] NR steps, [13] degenerate, [14] max NR error
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(kPointThreads) k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
+template <typename T, int WAVES>
+__global__ void __launch_bounds__(kPointThreads) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
+k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
                                                       double* __restrict__ sigma,
                                                       double* __restrict__ lambda, const double* __restrict__ R,
                                                       const double* __restrict__ crss,
@@ -237,53 +243,65 @@ __global__ void __launch_bounds__(kPointThreads) k_nr(long n, const T* __restric
   double acc[14];
 #pragma unroll
   for (int w = 0; w < 14; w++) acc[w] = 0.0;
-  double mx[1] = {0.0};
+  double mx[2] = {0.0, 0.0};  // max NR error, max NR steps
   for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
-    double r[9], it[kSlips], m[kSlips][5];
-    double d[5], s[5], lam[5];
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-      d[i] = ld(dtil + i * n + v);
-      s[i] = sigma[i * n + v];
-      lam[i] = lambda[i * n + v];
+    double it[kSlips], m[kSlips][5], s[5], q[5];
+    {
+      double r[9];
+      load_R(R, n, v, r);
+      schmid_from_R(r, m);
     }
-    load_R(R, n, v, r);
     load_inv_tau(crss, n, v, it);
-    schmid_from_R(r, m);
+#pragma unroll
+    for (int i = 0; i < 5; i++) s[i] = sigma[i * n + v];
     const double mrss = max_rss(s, m, it);
     if (!(mrss >= 1e-10)) acc[13] += 1.0;
 #pragma unroll
     for (int i = 0; i < 5; i++) s[i] /= mrss;
-    double q[5], Ll[5];
-    matvec5(C.L0inv, lam, Ll);
+    // q = L0^-1 lambda + edot + d~ ; lambda and d~ are re-read after the solve
 #pragma unroll
-    for (int i = 0; i < 5; i++) q[i] = Ll[i] + C.edot[i] + d[i];
+    for (int i = 0; i < 5; i++) q[i] = C.edot[i] + ld(dtil + i * n + v);
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      const double lj = lambda[j * n + v];
+#pragma unroll
+      for (int i = 0; i < 5; i++) q[i] += C.L0inv[j * 5 + i] * lj;
+    }
     double err = 0.0;
     int steps = 0;
-    constrained_nr(s, m, it, C, q, 5e-4, 1e9, 100, s, &err, &steps);
+    constrained_nr(s, m, it, C, q, 5e-4, 1e9, 100, &err, &steps);
     double e[5];
     apply_constitutive(s, m, it, C, e);
+    double d[5], lam[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      d[i] = ld(dtil + i * n + v);
+      lam[i] = lambda[i * n + v];
+    }
     double Ld[5], t[5], Lt[5];
     matvec5(C.L0, d, Ld);
 #pragma unroll
     for (int i = 0; i < 5; i++) t[i] = C.edot[i] + d[i] - e[i];
     matvec5(C.L0, t, Lt);
-    double se[5], ee[5];
+    double se2 = 0.0, ee2 = 0.0;
 #pragma unroll
     for (int i = 0; i < 5; i++) {
       lam[i] += Lt[i];
       phi[i * n + v] = (T)(s[i] - Ld[i]);
       sigma[i * n + v] = s[i];
       lambda[i * n + v] = lam[i];
-      se[i] = s[i] - lam[i];
-      ee[i] = e[i] - C.edot[i] - d[i];
+      const double se = s[i] - lam[i];
+      const double ee = e[i] - C.edot[i] - d[i];
+      se2 += se * se;
+      ee2 += ee * ee;
       acc[i] += s[i];
       acc[5 + i] += e[i];
     }
-    acc[10] += sqrt(dot5(se, se));
-    acc[11] += sqrt(dot5(ee, ee));
+    acc[10] += sqrt(se2);
+    acc[11] += sqrt(ee2);
     acc[12] += (double)steps;
     mx[0] = fmax(mx[0], err);
+    mx[1] = fmax(mx[1], (double)steps);
   }
   block_sum<14>(acc, mx, partial + (long)blockIdx.x * kPartialStride);
 }
@@ -291,19 +309,19 @@ __global__ void __launch_bounds__(kPointThreads) k_nr(long n, const T* __restric
 __global__ void __launch_bounds__(256) k_iter_finalize(long n, int nblocks, const double* __restrict__ partial,
                                                        const StepConst* __restrict__ C, double tol, int max_iter,
                                                        int it, IterRecord* rec) {
-  __shared__ double sh[256][15];
-  double acc[15];
-  for (int w = 0; w < 15; w++) acc[w] = 0.0;
+  __shared__ double sh[256][16];
+  double acc[16];
+  for (int w = 0; w < 16; w++) acc[w] = 0.0;
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
     for (int w = 0; w < 14; w++) acc[w] += partial[(long)b * kPartialStride + w];
-    acc[14] = fmax(acc[14], partial[(long)b * kPartialStride + 14]);
+    for (int w = 14; w < 16; w++) acc[w] = fmax(acc[w], partial[(long)b * kPartialStride + w]);
   }
-  for (int w = 0; w < 15; w++) sh[threadIdx.x][w] = acc[w];
+  for (int w = 0; w < 16; w++) sh[threadIdx.x][w] = acc[w];
   __syncthreads();
   for (int stride = 128; stride > 0; stride >>= 1) {
     if (threadIdx.x < stride) {
       for (int w = 0; w < 14; w++) sh[threadIdx.x][w] += sh[threadIdx.x + stride][w];
-      sh[threadIdx.x][14] = fmax(sh[threadIdx.x][14], sh[threadIdx.x + stride][14]);
+      for (int w = 14; w < 16; w++) sh[threadIdx.x][w] = fmax(sh[threadIdx.x][w], sh[threadIdx.x + stride][w]);
     }
     __syncthreads();
   }
@@ -327,6 +345,7 @@ __global__ void __launch_bounds__(256) k_iter_finalize(long n, int nblocks, cons
     R.nr_steps = sh[0][12];
     R.degenerate = sh[0][13];
     R.max_nr_error = sh[0][14];
+    R.max_nr_steps = sh[0][15];
     R.iteration = it + 1;
     // loop condition, MaterialGrid.cpp:1066-1068
     R.done = !((it + 1) < max_iter && (es > tol || ee > tol));
@@ -446,7 +465,7 @@ __global__ void k_constrained_batch(long n, const double* __restrict__ R, const 
     for (int i = 0; i < 5; i++) q[i] = Ll[i] + C.edot[i] + d[i];
     double e = err[v];
     int st = 0;
-    constrained_nr(s, m, it, C, q, 5e-4, 1e9, 100, s, &e, &st);
+    constrained_nr(s, m, it, C, q, 5e-4, 1e9, 100, &e, &st);
     for (int i = 0; i < 5; i++) s_out[5 * v + i] = s[i];
     err[v] = e;
     steps[v] = st;
@@ -492,15 +511,18 @@ hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenCon
   return hipGetLastError();
 }
 
-hipError_t launch_nr(int prec, long n, const void* dtil, void* phi, double* sigma, double* lambda, const double* R,
-                     const double* crss, const StepConst* C, double* partial, hipStream_t st) {
+hipError_t launch_nr(int prec, int waves, long n, const void* dtil, void* phi, double* sigma, double* lambda,
+                     const double* R, const double* crss, const StepConst* C, double* partial, hipStream_t st) {
   const int nb = point_blocks(n);
-  if (prec == 0)
-    hipLaunchKernelGGL(k_nr<double>, dim3(nb), dim3(kPointThreads), 0, st, n, (const double*)dtil, (double*)phi,
-                       sigma, lambda, R, crss, C, partial);
-  else
-    hipLaunchKernelGGL(k_nr<float>, dim3(nb), dim3(kPointThreads), 0, st, n, (const float*)dtil, (float*)phi,
-                       sigma, lambda, R, crss, C, partial);
+#define NR_LAUNCH(TT, WW)                                                                                   \
+  hipLaunchKernelGGL((k_nr<TT, WW>), dim3(nb), dim3(kPointThreads), 0, st, n, (const TT*)dtil, (TT*)phi, sigma, \
+                     lambda, R, crss, C, partial)
+  if (prec == 0) {
+    if (waves >= 2) NR_LAUNCH(double, 2); else NR_LAUNCH(double, 1);
+  } else {
+    if (waves >= 2) NR_LAUNCH(float, 2); else NR_LAUNCH(float, 1);
+  }
+#undef NR_LAUNCH
   return hipGetLastError();
 }
 
