@@ -67,7 +67,7 @@ static const char* kKernelNames[kNumKernels] = {"prologue", "L0_finalize", "fft_
                                                 "iter_finalize", "epilogue"};
 
 struct Grid {
-  int nx = 0, ny = 0, nz = 0, nzh = 0;
+  int nx = 0, ny = 0, nz = 0, nzh = 0, nzp = 0;
   long n = 0, nc = 0;
   int prec = 0;
   double lx = 1, ly = 1, lz = 1;
@@ -94,6 +94,10 @@ struct Grid {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool uploaded = false;
   int nr_waves = 2;  // occupancy variant of the Newton kernel (VPFFT_NR_WAVES=1|2)
+  // every voxel's 12 CRSS values are equal (uploaded uniform and kept so by a
+  // hardening law whose matrix rows and Voce constants are slip-independent)
+  bool crss_uniform = false;
+  bool hardening_uniform = false;
   int phi_prec = 0, dtil_prec = 0;  // precision the field buffers currently hold
   // per-kernel timing with HIP events on the grid's stream
   bool profile = false;
@@ -308,6 +312,14 @@ static int download_field5(Grid* g, const void* dev, double* host, int prec) {
   return VPFFT_OK;
 }
 
+// true when every voxel's 12 CRSS values are bitwise equal
+static bool crss_rows_uniform(const double* crss, long n) {
+  for (long v = 0; v < n; v++)
+    for (int k = 1; k < kSlips; k++)
+      if (crss[12 * v + k] != crss[12 * v]) return false;
+  return true;
+}
+
 static int check_material(const vpfft_material* m) {
   for (int k = 0; k < kSlips; k++) {
     if (!(m->gamma_dot_base[k] > 0) || !std::isfinite(m->gamma_dot_base[k]))
@@ -332,11 +344,20 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
   g->ny = d->ny;
   g->nz = d->nz;
   g->nzh = d->nz / 2 + 1;
+  g->nzp = padded_half(d->nz);
   g->n = (long)d->nx * d->ny * d->nz;
-  g->nc = (long)d->nx * d->ny * g->nzh;
+  g->nc = (long)d->nx * d->ny * g->nzp;
   g->prec = d->precision;
   g->mat = *mat;
   if (const char* w = std::getenv("VPFFT_NR_WAVES")) g->nr_waves = std::atoi(w) >= 2 ? 2 : 1;
+  g->hardening_uniform = true;
+  for (int i = 0; i < kSlips; i++) {
+    if (mat->tau0[i] != mat->tau0[0] || mat->tau1[i] != mat->tau1[0] || mat->theta0[i] != mat->theta0[0] ||
+        mat->theta1[i] != mat->theta1[0])
+      g->hardening_uniform = false;
+    for (int j = 0; j < kSlips; j++)
+      if (mat->hardening[i * kSlips + j] != mat->hardening[j]) g->hardening_uniform = false;
+  }
   const int dims[3] = {g->nx, g->ny, g->nz};
   for (int a = 0; a < 3; a++)
     if (!make_axis_plan(dims[a], &g->plan[a])) {
@@ -344,8 +365,8 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
       return set_error(VPFFT_E_UNSUPPORTED, "grid edge %d has a prime factor other than 2, 3, 5, 7", dims[a]);
     }
   // LDS limits of the FFT passes (2 ping-pong buffers of one pencil batch in FP64)
-  if ((long)3 * g->nz * 16 * 2 > 160 * 1024 || (long)g->ny * 16 * 2 > 160 * 1024 ||
-      (long)g->nx * 16 * 2 > 160 * 1024) {
+  if ((long)4 * g->nz * 16 * 2 > 160 * 1024 || (long)g->ny * 16 * 2 > 160 * 1024 ||
+      (long)g->nx * 16 * 8 * 2 > 160 * 1024) {
     grid_free(g);
     return set_error(VPFFT_E_UNSUPPORTED, "grid edge too long for the LDS-resident FFT");
   }
@@ -356,6 +377,7 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
   g->geom.ny = g->ny;
   g->geom.nz = g->nz;
   g->geom.nzh = g->nzh;
+  g->geom.nzp = g->nzp;
   hipError_t e = hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking);
   if (e != hipSuccess) {
     grid_free(g);
@@ -391,7 +413,7 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
   ALLOC(g->dC, sizeof(StepConst));
   ALLOC(g->dG64, sizeof(GreenConst<double>));
   ALLOC(g->dG32, sizeof(GreenConst<float>));
-  ALLOC(g->partial, sizeof(double) * 4096 * 18);
+  ALLOC(g->partial, sizeof(double) * std::max<long>(4096L * 18, nr_partial_rows(n) * kPartialStride));
   ALLOC(g->dprol, sizeof(PrologueRecord));
   g->staging_elems = std::min<long>(60L * n, 8L << 20);
   ALLOC(g->staging, sizeof(double) * g->staging_elems);
@@ -447,8 +469,8 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
   for (;;) {
     if ((rc = run_fft_green(g, prec))) return rc;
     kbegin(g, K_NR);
-    HIP_TRY(launch_nr(prec, g->nr_waves, n, g->dtil, g->phi, g->sigma, g->lambda, g->R, g->crss, g->dC, g->partial,
-                      g->st));
+    HIP_TRY(launch_nr(prec, g->nr_waves, g->crss_uniform, n, g->dtil, g->phi, g->sigma, g->lambda, g->R, g->crss,
+                      g->dC, g->partial, g->st));
     kend(g, K_NR);
     g->phi_prec = prec;
     kbegin(g, K_ITER_FINALIZE);
@@ -473,6 +495,7 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
     kbegin(g, K_EPILOGUE);
     HIP_TRY(launch_epilogue(n, g->R, g->crss, g->acc, g->sigma, g->dC, g->st));
     kend(g, K_EPILOGUE);
+    g->crss_uniform = g->crss_uniform && g->hardening_uniform;
     double d[3];
     edot_diag(edot, d);
     g->lx *= (1.0 + d[0] * dt);
@@ -602,7 +625,9 @@ int vpfft_grid_upload(void* grid, const double* orientation, const double* crss,
   if ((rc = vpfft::upload_aos(g, orientation, g->R, 9))) return rc;
   if (crss) {
     if ((rc = vpfft::upload_aos(g, crss, g->crss, 12))) return rc;
+    g->crss_uniform = vpfft::crss_rows_uniform(crss, g->n);
   } else {
+    g->crss_uniform = true;
     if (!(crss_value > 0)) return set_error(VPFFT_E_INVALID, "crss_value must be positive");
     hipLaunchKernelGGL(vpfft::k_fill, dim3(vpfft::fill_blocks(12 * g->n)), dim3(256), 0, g->st, g->crss, crss_value,
                        12 * g->n);
@@ -625,7 +650,9 @@ int vpfft_grid_upload_field(void* grid, int field, const double* host) {
   if (!g || !host) return set_error(VPFFT_E_INVALID, "null argument");
   switch (field) {
     case VPFFT_FIELD_ORIENTATION: return vpfft::upload_aos(g, host, g->R, 9);
-    case VPFFT_FIELD_CRSS: return vpfft::upload_aos(g, host, g->crss, 12);
+    case VPFFT_FIELD_CRSS:
+      g->crss_uniform = vpfft::crss_rows_uniform(host, g->n);
+      return vpfft::upload_aos(g, host, g->crss, 12);
     case VPFFT_FIELD_STRESS: return vpfft::upload_aos(g, host, g->sigma, 5);
     case VPFFT_FIELD_LAGRANGE: return vpfft::upload_aos(g, host, g->lambda, 5);
     case VPFFT_FIELD_POLARIZATION: g->phi_prec = g->prec; return vpfft::upload_field5(g, host, g->phi, g->prec);

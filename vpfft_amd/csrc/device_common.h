@@ -657,11 +657,22 @@ __device__ __forceinline__ void taylor_nr(const double init[5], const double m[k
       const double g = (n - 1 > 0) ? C.gdot0[k] * ipow_cr(fabs(rss[k]), n - 1) : C.gdot0[k];
 #pragma unroll
       for (int i = 0; i < 5; i++) res[i] += m[k][i] * g * rss[k];
+      // J(i,j) and J(j,i) receive bit-identical increments (m_i m_j == m_j m_i),
+      // so the lower triangle is computed and mirrored; x / 1.0 == x exactly,
+      // so the division is skipped when tau is 1
+      const bool unit_tau = tau[k] == 1.0;
 #pragma unroll
       for (int j = 0; j < 5; j++)
 #pragma unroll
-        for (int i = 0; i < 5; i++) J[j * 5 + i] -= m[k][i] * m[k][j] * (double)n * g / tau[k];
+        for (int i = j; i < 5; i++) {
+          const double t = m[k][i] * m[k][j] * (double)n * g;
+          J[j * 5 + i] -= unit_tau ? t : t / tau[k];
+        }
     }
+#pragma unroll
+    for (int j = 0; j < 5; j++)
+#pragma unroll
+      for (int i = 0; i < j; i++) J[j * 5 + i] = J[i * 5 + j];
 #pragma unroll
     for (int i = 0; i < 5; i++) res[i] = res[i] - C.edot[i];
 #pragma unroll

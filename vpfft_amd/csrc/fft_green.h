@@ -6,10 +6,14 @@ namespace vpfft {
 
 struct FftGeom {
   long n;      // voxels per component (Nx*Ny*Nz)
-  long nc;     // half-spectrum bins per component (Nx*Ny*(Nz/2+1))
+  long nc;     // spectrum stride per component (Nx*Ny*nzp)
   long nrows;  // Nx*Ny
   int nx, ny, nz, nzh;
+  int nzp;     // spectrum row stride: Nz/2+1 rounded up to 16, so every
+               // row starts on a 128-byte line in FP64 and FP32
 };
+
+inline int padded_half(int nz) { return ((nz / 2 + 1) + 15) / 16 * 16; }
 
 struct AxisPlan {
   int n;

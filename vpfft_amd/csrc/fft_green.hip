@@ -7,7 +7,9 @@
 // Layout in HBM (component-major, like the reference's one buffer per
 // component, MaterialGrid.h:320-321):
 //   real field   f[c][x][y][z]            c = 0..4, N = Nx*Ny*Nz per component
-//   spectrum     s[c][x][y][kz]           kz = 0..Nz/2, Nc = Nx*Ny*(Nz/2+1)
+//   spectrum     s[c][x][y][kz]           kz = 0..Nz/2 in rows of nzp >= Nz/2+1
+//                (padded to a multiple of 16 so runs of kz start on 128-byte
+//                lines; the pad bins are never read or written)
 // Passes per fixed-point iteration:
 //   z_r2c : f -> s   (two real components packed in one complex FFT)
 //   y     : s in place, forward
@@ -129,8 +131,9 @@ __device__ __forceinline__ void dft(typename C2<T>::t* v, int sign) {
   }
 }
 
-// One Stockham stage over P pencils held in LDS as buf[j * P + p].
-template <typename T, int P, int R>
+// One Stockham stage over P pencils held in LDS as buf[j * LS + p], LS >= P
+// (an odd LS breaks the bank pattern of the row-transposing loads).
+template <typename T, int P, int R, int LS = P>
 __device__ __forceinline__ void stockham_stage(const typename C2<T>::t* __restrict__ in,
                                                typename C2<T>::t* __restrict__ out, int n, int ns,
                                                const typename C2<T>::t* __restrict__ tw, int sign) {
@@ -143,7 +146,7 @@ __device__ __forceinline__ void stockham_stage(const typename C2<T>::t* __restri
     const int j = u / P;
     C v[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) v[r] = in[(j + r * nb) * P + p];
+    for (int r = 0; r < R; r++) v[r] = in[(j + r * nb) * LS + p];
     const int jm = j % ns;
     if (ns > 1) {
 #pragma unroll
@@ -156,13 +159,13 @@ __device__ __forceinline__ void stockham_stage(const typename C2<T>::t* __restri
     dft<T, R>(v, sign);
     const int idxd = (j - jm) * R + jm;
 #pragma unroll
-    for (int r = 0; r < R; r++) out[(idxd + r * ns) * P + p] = v[r];
+    for (int r = 0; r < R; r++) out[(idxd + r * ns) * LS + p] = v[r];
   }
 }
 
 // Full transform of P pencils of length n.  Data starts in buf0; returns the
 // buffer holding the result (buf0 or buf1).  Ends with a barrier.
-template <typename T, int P>
+template <typename T, int P, int LS = P>
 __device__ typename C2<T>::t* lds_fft(typename C2<T>::t* buf0, typename C2<T>::t* buf1,
                                       const AxisPlan& ap, const typename C2<T>::t* tw, int sign) {
   using C = typename C2<T>::t;
@@ -172,12 +175,12 @@ __device__ typename C2<T>::t* lds_fft(typename C2<T>::t* buf0, typename C2<T>::t
   for (int s = 0; s < ap.nstages; s++) {
     const int r = ap.radix[s];
     switch (r) {
-      case 2: stockham_stage<T, P, 2>(in, out, ap.n, ns, tw, sign); break;
-      case 3: stockham_stage<T, P, 3>(in, out, ap.n, ns, tw, sign); break;
-      case 4: stockham_stage<T, P, 4>(in, out, ap.n, ns, tw, sign); break;
-      case 5: stockham_stage<T, P, 5>(in, out, ap.n, ns, tw, sign); break;
-      case 7: stockham_stage<T, P, 7>(in, out, ap.n, ns, tw, sign); break;
-      default: stockham_stage<T, P, 8>(in, out, ap.n, ns, tw, sign); break;
+      case 2: stockham_stage<T, P, 2, LS>(in, out, ap.n, ns, tw, sign); break;
+      case 3: stockham_stage<T, P, 3, LS>(in, out, ap.n, ns, tw, sign); break;
+      case 4: stockham_stage<T, P, 4, LS>(in, out, ap.n, ns, tw, sign); break;
+      case 5: stockham_stage<T, P, 5, LS>(in, out, ap.n, ns, tw, sign); break;
+      case 7: stockham_stage<T, P, 7, LS>(in, out, ap.n, ns, tw, sign); break;
+      default: stockham_stage<T, P, 8, LS>(in, out, ap.n, ns, tw, sign); break;
     }
     __syncthreads();
     ns *= r;
@@ -197,12 +200,13 @@ __global__ void __launch_bounds__(256) k_z_r2c(const T* __restrict__ f, typename
                                                FftGeom g, AxisPlan ap, const typename C2<T>::t* __restrict__ tw) {
   using C = typename C2<T>::t;
   constexpr int P = 3 * B;
+  constexpr int LS = P;
   extern __shared__ unsigned char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
-  C* buf1 = buf0 + P * g.nz;
+  C* buf1 = buf0 + LS * g.nz;
   const long row0 = (long)blockIdx.x * B;
   const int nrows = (int)min((long)B, g.nrows - row0);
-  const int nz = g.nz, nzh = g.nzh;
+  const int nz = g.nz, nzh = g.nzh, nzp = g.nzp;
   const int cnt = B * nz;
   for (int pair = 0; pair < 3; pair++) {
     const T* fa = f + (long)(2 * pair) * g.n + row0 * nz;
@@ -214,25 +218,25 @@ __global__ void __launch_bounds__(256) k_z_r2c(const T* __restrict__ f, typename
         a = fa[e];
         if (fb) c = fb[e];
       }
-      buf0[j * P + pair * B + b] = cmake<T>(a, c);
+      buf0[j * LS + pair * B + b] = cmake<T>(a, c);
     }
   }
   __syncthreads();
-  C* res = lds_fft<T, P>(buf0, buf1, ap, tw, -1);
+  C* res = lds_fft<T, P, LS>(buf0, buf1, ap, tw, -1);
   const int cnth = B * nzh;
   for (int pair = 0; pair < 3; pair++) {
-    C* sa = s + (long)(2 * pair) * g.nc + row0 * nzh;
-    C* sb = s + (long)(2 * pair + 1) * g.nc + row0 * nzh;
+    C* sa = s + (long)(2 * pair) * g.nc + row0 * nzp;
+    C* sb = s + (long)(2 * pair + 1) * g.nc + row0 * nzp;
     for (int e = threadIdx.x; e < cnth; e += blockDim.x) {
       const int b = e / nzh, k = e - b * nzh;
       if (b >= nrows) continue;
-      const C zk = res[k * P + pair * B + b];
+      const C zk = res[k * LS + pair * B + b];
       const int kn = k == 0 ? 0 : nz - k;
-      const C zn = res[kn * P + pair * B + b];
+      const C zn = res[kn * LS + pair * B + b];
       // X_a = (Z_k + conj Z_{-k}) / 2 ;  X_b = -i/2 (Z_k - conj Z_{-k})
       const T h = (T)0.5;
-      sa[e] = cmake<T>(h * (zk.x + zn.x), h * (zk.y - zn.y));
-      if (pair < 2) sb[e] = cmake<T>(h * (zk.y + zn.y), -h * (zk.x - zn.x));
+      sa[b * nzp + k] = cmake<T>(h * (zk.x + zn.x), h * (zk.y - zn.y));
+      if (pair < 2) sb[b * nzp + k] = cmake<T>(h * (zk.y + zn.y), -h * (zk.x - zn.x));
     }
   }
 }
@@ -246,33 +250,34 @@ __global__ void __launch_bounds__(256) k_z_c2r(const typename C2<T>::t* __restri
                                                T scale) {
   using C = typename C2<T>::t;
   constexpr int P = 3 * B;
+  constexpr int LS = P;
   extern __shared__ unsigned char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
-  C* buf1 = buf0 + P * g.nz;
+  C* buf1 = buf0 + LS * g.nz;
   const long row0 = (long)blockIdx.x * B;
   const int nrows = (int)min((long)B, g.nrows - row0);
-  const int nz = g.nz, nzh = g.nzh;
+  const int nz = g.nz, nzh = g.nzh, nzp = g.nzp;
   const int cnth = B * nzh;
   const bool even = (nz & 1) == 0;
   for (int pair = 0; pair < 3; pair++) {
-    const C* sa = s + (long)(2 * pair) * g.nc + row0 * nzh;
-    const C* sb = s + (long)(2 * pair + 1) * g.nc + row0 * nzh;
+    const C* sa = s + (long)(2 * pair) * g.nc + row0 * nzp;
+    const C* sb = s + (long)(2 * pair + 1) * g.nc + row0 * nzp;
     for (int e = threadIdx.x; e < cnth; e += blockDim.x) {
       const int b = e / nzh, k = e - b * nzh;
       C xa = cmake<T>(0, 0), xb = cmake<T>(0, 0);
       if (b < nrows) {
-        xa = sa[e];
-        if (pair < 2) xb = sb[e];
+        xa = sa[b * nzp + k];
+        if (pair < 2) xb = sb[b * nzp + k];
       }
       // C2R reads only the real part of the self-conjugate bins
       if (k == 0 || (even && 2 * k == nz)) { xa.y = 0; xb.y = 0; }
       // Z_k = X_a + i X_b ;  Z_{-k} = conj X_a + i conj X_b
-      buf0[k * P + pair * B + b] = cmake<T>(xa.x - xb.y, xa.y + xb.x);
-      if (k > 0 && 2 * k != nz) buf0[(nz - k) * P + pair * B + b] = cmake<T>(xa.x + xb.y, xb.x - xa.y);
+      buf0[k * LS + pair * B + b] = cmake<T>(xa.x - xb.y, xa.y + xb.x);
+      if (k > 0 && 2 * k != nz) buf0[(nz - k) * LS + pair * B + b] = cmake<T>(xa.x + xb.y, xb.x - xa.y);
     }
   }
   __syncthreads();
-  C* res = lds_fft<T, P>(buf0, buf1, ap, tw, +1);
+  C* res = lds_fft<T, P, LS>(buf0, buf1, ap, tw, +1);
   const int cnt = B * nz;
   for (int pair = 0; pair < 3; pair++) {
     T* fa = f + (long)(2 * pair) * g.n + row0 * nz;
@@ -280,7 +285,7 @@ __global__ void __launch_bounds__(256) k_z_c2r(const typename C2<T>::t* __restri
     for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
       const int b = e / nz, j = e - b * nz;
       if (b >= nrows) continue;
-      const C z = res[j * P + pair * B + b];
+      const C z = res[j * LS + pair * B + b];
       fa[e] = z.x * scale;
       if (pair < 2) fb[e] = z.y * scale;
     }
@@ -302,17 +307,17 @@ __global__ void __launch_bounds__(256) k_y(typename C2<T>::t* __restrict__ s, Ff
   const long plane = blockIdx.x / runs;  // c * nx + x
   const int kz0 = run * B;
   const int nb = min(B, g.nzh - kz0);
-  C* base = s + plane * (long)g.ny * g.nzh + kz0;
+  C* base = s + plane * (long)g.ny * g.nzp + kz0;
   const int cnt = B * g.ny;
   for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
     const int b = e % B, y = e / B;
-    buf0[e] = b < nb ? base[(long)y * g.nzh + b] : cmake<T>(0, 0);
+    buf0[e] = b < nb ? base[(long)y * g.nzp + b] : cmake<T>(0, 0);
   }
   __syncthreads();
   C* res = lds_fft<T, B>(buf0, buf1, ap, tw, sign);
   for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
     const int b = e % B, y = e / B;
-    if (b < nb) base[(long)y * g.nzh + b] = res[e];
+    if (b < nb) base[(long)y * g.nzp + b] = res[e];
   }
 }
 
@@ -419,18 +424,23 @@ __device__ __forceinline__ void green_apply(typename C2<T>::t ph[5], int ix, int
 // frequencies kept in registers (E per thread), Green applied, inverse FFT.
 // ---------------------------------------------------------------------------
 template <typename T, int B, int E>
-__global__ void __launch_bounds__(256) k_x_green(typename C2<T>::t* __restrict__ s, FftGeom g, AxisPlan ap,
+__global__ void __launch_bounds__(512) k_x_green(typename C2<T>::t* __restrict__ s, FftGeom g, AxisPlan ap,
                                                  const typename C2<T>::t* __restrict__ tw,
                                                  const GreenConst<T>* __restrict__ gc) {
   using C = typename C2<T>::t;
   extern __shared__ unsigned char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
   C* buf1 = buf0 + B * g.nx;
-  const long ncol = (long)g.ny * g.nzh;
-  const long q0 = (long)blockIdx.x * B;
-  const int nb = (int)min((long)B, ncol - q0);
+  const long ncol = (long)g.ny * g.nzp;  // x stride
+  const int runs = (g.nzh + B - 1) / B;
+  const int iy = blockIdx.x / runs;
+  const int kz0 = (blockIdx.x - iy * runs) * B;
+  const long q0 = (long)iy * g.nzp + kz0;
+  const int nb = min(B, g.nzh - kz0);
   const int cnt = B * g.nx;
   C reg[5][E];
+  // component loops are unrolled so reg[][] stays in registers
+#pragma unroll
   for (int c = 0; c < 5; c++) {
     C* base = s + (long)c * g.nc + q0;
     for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
@@ -452,8 +462,7 @@ __global__ void __launch_bounds__(256) k_x_green(typename C2<T>::t* __restrict__
     const int e = threadIdx.x + i * blockDim.x;
     const int b = e % B, x = e / B;
     if (e < cnt && b < nb) {
-      const long q = q0 + b;
-      const int iy = (int)(q / g.nzh), iz = (int)(q - (long)iy * g.nzh);
+      const int iz = kz0 + b;
       C ph[5];
 #pragma unroll
       for (int c = 0; c < 5; c++) ph[c] = reg[c][i];
@@ -462,6 +471,7 @@ __global__ void __launch_bounds__(256) k_x_green(typename C2<T>::t* __restrict__
       for (int c = 0; c < 5; c++) reg[c][i] = ph[c];
     }
   }
+#pragma unroll
   for (int c = 0; c < 5; c++) {
 #pragma unroll
     for (int i = 0; i < E; i++) {
@@ -559,25 +569,27 @@ template <typename T>
 static hipError_t launch_x_green_t(void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
                                    const void* gc, hipStream_t st) {
   using C = typename C2<T>::t;
-  // keeps 5 * E complex values per thread in registers: E <= 4
-  const int B = pick_run(g.nx, 1024);
-  const long ncol = (long)g.ny * g.nzh;
-  const unsigned grid = (unsigned)((ncol + B - 1) / B);
+  // runs of at least 128 bytes along kz (8 complex doubles, 16 complex
+  // floats); 512 threads keep 5 * E complex values per thread in registers
+  const int minrun = sizeof(T) == 8 ? 8 : 16;
+  int B = pick_run(g.nx, 2048);
+  if (B < minrun) B = minrun;
+  const int runs = (g.nzh + B - 1) / B;
+  const unsigned grid = (unsigned)((long)g.ny * runs);
   const size_t lds = 2 * sizeof(C) * B * g.nx;
   const int cnt = B * g.nx;
-  const int E = (cnt + 255) / 256;
+  const int E = (cnt + 511) / 512;
   const GreenConst<T>* G = (const GreenConst<T>*)gc;
-#define LAUNCH(BB, EE) hipLaunchKernelGGL((k_x_green<T, BB, EE>), dim3(grid), dim3(256), lds, st, (C*)s, g, ap, (const C*)tw, G)
+#define LAUNCH(BB, EE) hipLaunchKernelGGL((k_x_green<T, BB, EE>), dim3(grid), dim3(512), lds, st, (C*)s, g, ap, (const C*)tw, G)
 #define EC(BB)                         \
   case BB:                             \
     if (E <= 1) LAUNCH(BB, 1);         \
     else if (E <= 2) LAUNCH(BB, 2);    \
     else if (E <= 4) LAUNCH(BB, 4);    \
-    else if (E <= 8) LAUNCH(BB, 8);    \
-    else LAUNCH(BB, 16);               \
+    else LAUNCH(BB, 8);                \
     break;
   switch (B) {
-    EC(1) EC(2) EC(4) EC(8) EC(16) EC(32) default: EC(64)
+    EC(8) EC(16) EC(32) default: EC(64)
   }
 #undef EC
 #undef LAUNCH

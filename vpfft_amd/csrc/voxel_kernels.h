@@ -12,6 +12,11 @@ namespace vpfft {
 constexpr int kPointThreads = 256;
 constexpr int kPartialStride = 16;
 
+// the Newton kernel runs one voxel per lane
+inline int nr_blocks(long n) { return (int)((n + kPointThreads - 1) / kPointThreads); }
+// partial-sum buffer rows needed by launch_nr (+ its folded rows)
+inline long nr_partial_rows(long n) { return nr_blocks(n) + (nr_blocks(n) + 255) / 256; }
+
 inline int point_blocks(long n) {
   long b = (n + kPointThreads - 1) / kPointThreads;
   if (b > 4096) b = 4096;
@@ -45,9 +50,11 @@ hipError_t launch_L0_finalize(long n, int nblocks, const double* partial, StepCo
                               GreenConst<float>* g32, PrologueRecord* rec, hipStream_t st);
 // derive L0^-1 and the Green constants from C->L0 (already set)
 hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32, hipStream_t st);
-// waves: 1 or 2 waves per SIMD (register budget 512 or 256 per lane)
-hipError_t launch_nr(int prec, int waves, long n, const void* dtil, void* phi, double* sigma, double* lambda,
-                     const double* R, const double* crss, const StepConst* C, double* partial, hipStream_t st);
+// waves: 1 or 2 waves per SIMD (register budget 512 or 256 per lane);
+// uniform_tau: every slip system of a voxel has the same CRSS (crss[0][v] is read)
+hipError_t launch_nr(int prec, int waves, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
+                     double* lambda, const double* R, const double* crss, const StepConst* C, double* partial,
+                     hipStream_t st);
 hipError_t launch_iter_finalize(long n, int nblocks, const double* partial, const StepConst* C, double tol,
                                 int max_iter, int it, IterRecord* rec, hipStream_t st);
 hipError_t launch_epilogue(long n, double* R, double* crss, double* acc, const double* sigma, const StepConst* C,

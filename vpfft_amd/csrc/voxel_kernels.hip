@@ -231,7 +231,7 @@ __global__ void k_derive(StepConst* C, GreenConst<double>* g64, GreenConst<float
 // partial per block: [0..4] sum sigma, [5..9] sum edot, [10] sum|sigma-lambda|,
 // [11] sum|edot-E-d|, [12] NR steps, [13] degenerate, [14] max NR error
 // ---------------------------------------------------------------------------
-template <typename T, int WAVES>
+template <typename T, int WAVES, bool UNIFORM_TAU>
 __global__ void __launch_bounds__(kPointThreads) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
 k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
                                                       double* __restrict__ sigma,
@@ -244,14 +244,23 @@ k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
 #pragma unroll
   for (int w = 0; w < 14; w++) acc[w] = 0.0;
   double mx[2] = {0.0, 0.0};  // max NR error, max NR steps
-  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
+  // one voxel per lane: nothing but the voxel's own state is live
+  const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) {
     double it[kSlips], m[kSlips][5], s[5], q[5];
     {
       double r[9];
       load_R(R, n, v, r);
       schmid_from_R(r, m);
     }
-    load_inv_tau(crss, n, v, it);
+    if (UNIFORM_TAU) {
+      // every slip system of the voxel has the same CRSS (one load)
+      const double t = 1.0 / crss[v];
+#pragma unroll
+      for (int k = 0; k < kSlips; k++) it[k] = t;
+    } else {
+      load_inv_tau(crss, n, v, it);
+    }
 #pragma unroll
     for (int i = 0; i < 5; i++) s[i] = sigma[i * n + v];
     const double mrss = max_rss(s, m, it);
@@ -304,6 +313,25 @@ k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
     mx[1] = fmax(mx[1], (double)steps);
   }
   block_sum<14>(acc, mx, partial + (long)blockIdx.x * kPartialStride);
+}
+
+// Reduce rows [256 b, 256 b + 256) of the per-block partials into row b of
+// out: 14 sums then 2 maxima, fixed tree order.
+__global__ void __launch_bounds__(256) k_fold_partials(const double* __restrict__ in, int rows,
+                                                       double* __restrict__ out) {
+  __shared__ double sh[256][16];
+  const long r = (long)blockIdx.x * 256 + threadIdx.x;
+  for (int w = 0; w < 16; w++) sh[threadIdx.x][w] = r < rows ? in[r * kPartialStride + w] : 0.0;
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if (threadIdx.x < stride) {
+      for (int w = 0; w < 14; w++) sh[threadIdx.x][w] += sh[threadIdx.x + stride][w];
+      for (int w = 14; w < 16; w++) sh[threadIdx.x][w] = fmax(sh[threadIdx.x][w], sh[threadIdx.x + stride][w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int w = 0; w < 16; w++) out[(long)blockIdx.x * kPartialStride + w] = sh[0][w];
 }
 
 __global__ void __launch_bounds__(256) k_iter_finalize(long n, int nblocks, const double* __restrict__ partial,
@@ -511,24 +539,39 @@ hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenCon
   return hipGetLastError();
 }
 
-hipError_t launch_nr(int prec, int waves, long n, const void* dtil, void* phi, double* sigma, double* lambda,
-                     const double* R, const double* crss, const StepConst* C, double* partial, hipStream_t st) {
-  const int nb = point_blocks(n);
-#define NR_LAUNCH(TT, WW)                                                                                   \
-  hipLaunchKernelGGL((k_nr<TT, WW>), dim3(nb), dim3(kPointThreads), 0, st, n, (const TT*)dtil, (TT*)phi, sigma, \
+hipError_t launch_nr(int prec, int waves, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
+                     double* lambda, const double* R, const double* crss, const StepConst* C, double* partial,
+                     hipStream_t st) {
+  const int nb = nr_blocks(n);
+#define NR_LAUNCH(TT, WW, UU)                                                                                    \
+  hipLaunchKernelGGL((k_nr<TT, WW, UU>), dim3(nb), dim3(kPointThreads), 0, st, n, (const TT*)dtil, (TT*)phi, sigma, \
                      lambda, R, crss, C, partial)
+#define NR_W(TT, UU)                   \
+  if (waves >= 2) NR_LAUNCH(TT, 2, UU); \
+  else NR_LAUNCH(TT, 1, UU);
   if (prec == 0) {
-    if (waves >= 2) NR_LAUNCH(double, 2); else NR_LAUNCH(double, 1);
+    if (uniform_tau) { NR_W(double, true) } else { NR_W(double, false) }
   } else {
-    if (waves >= 2) NR_LAUNCH(float, 2); else NR_LAUNCH(float, 1);
+    if (uniform_tau) { NR_W(float, true) } else { NR_W(float, false) }
   }
+#undef NR_W
 #undef NR_LAUNCH
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // fold the per-block partials in a fixed order: nb -> ceil(nb / 256) rows
+  const int nb2 = (nb + 255) / 256;
+  hipLaunchKernelGGL(k_fold_partials, dim3(nb2), dim3(256), 0, st, partial, nb, partial + (long)nb * kPartialStride);
   return hipGetLastError();
 }
 
 hipError_t launch_iter_finalize(long n, int nblocks, const double* partial, const StepConst* C, double tol,
                                 int max_iter, int it, IterRecord* rec, hipStream_t st) {
-  hipLaunchKernelGGL(k_iter_finalize, dim3(1), dim3(256), 0, st, n, nblocks, partial, C, tol, max_iter, it, rec);
+  // launch_nr left ceil(nr_blocks / 256) folded rows after the per-block partials
+  const int nb = nr_blocks(n);
+  const int nb2 = (nb + 255) / 256;
+  (void)nblocks;
+  hipLaunchKernelGGL(k_iter_finalize, dim3(1), dim3(256), 0, st, n, nb2, partial + (long)nb * kPartialStride, C,
+                     tol, max_iter, it, rec);
   return hipGetLastError();
 }
 
