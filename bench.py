@@ -46,26 +46,39 @@ def fft_bytes(kernel, n, nc, elt):
 
 
 def run_form(V, R, mat, edot, args, precision, steps, warmup, profile):
+    """Warm up, then time `steps` strain steps from the initial state.  The
+    timed steps replay the fixed-point loop as captured HIP graphs.  With
+    profile=True the same steps run once more from the same initial state with
+    every kernel launched from the host between HIP events on the grid's
+    stream, which gives the per-kernel durations."""
     n = R.shape[0]
     g = V.MaterialGrid(args.size, args.size, args.size, mat, precision=precision)
     g.SendDataAndInitialize(R)
     for _ in range(warmup):
         g.RunSingleStrainStep(edot, args.max_iter, args.dt, args.tol)
-    # restart from the initial state so both forms time the same steps
-    g.SendDataAndInitialize(R)
-    g.SetVoxelLength(1.0, 1.0, 1.0)
+
+    def timed(prof):
+        # restart from the initial state so every pass times the same steps
+        g.SendDataAndInitialize(R)
+        g.SetVoxelLength(1.0, 1.0, 1.0)
+        g.SetProfiling(prof)
+        iters, avgs, infos = 0, [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            avgs.append(g.RunSingleStrainStep(edot, args.max_iter, args.dt, args.tol))
+            infos.append(g.last_info.as_dict())
+            iters += g.last_info.iterations
+        wall = time.perf_counter() - t0
+        return dict(wall=wall, iters=iters, vox_iters=n * iters, avgs=avgs, infos=infos)
+
+    out = timed(False)
     if profile:
-        g.SetProfiling(True)
-    iters, avgs, infos = 0, [], []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        avgs.append(g.RunSingleStrainStep(edot, args.max_iter, args.dt, args.tol))
-        infos.append(g.last_info.as_dict())
-        iters += g.last_info.iterations
-    wall = time.perf_counter() - t0
-    ktimes = g.KernelTimes() if profile else {}
+        p = timed(True)
+        out["ktimes"] = g.KernelTimes()
+        out["profiled_wall"] = p["wall"]
+        out["profiled_same_bits"] = all(np.array_equal(a, b) for a, b in zip(out["avgs"], p["avgs"]))
     g.close()
-    return dict(wall=wall, iters=iters, vox_iters=n * iters, avgs=avgs, infos=infos, ktimes=ktimes)
+    return out
 
 
 def cpu_baseline(V, args, edot):
@@ -121,6 +134,9 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=96)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-max-iters", type=int, default=20)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first copy this many MiB device to device (torch) three times: a kernel of known "
+                         "traffic that scales FETCH_SIZE / WRITE_SIZE counters read under rocprofv3")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -131,6 +147,16 @@ def main():
 
     import vpfft_amd as V
     V.load_library()
+
+    if args.copy_probe_mib > 0:
+        import torch
+        a = torch.ones(args.copy_probe_mib << 17, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: 3 x {args.copy_probe_mib} MiB read + written")
+        del a, b
 
     log(f"[bench] generating {args.size}^3 microstructure, {args.grains} grains")
     R, _, _ = V.generate_voronoi(args.size, args.size, args.size, args.grains, args.seed)
@@ -145,7 +171,7 @@ def main():
     log(f"[bench] FP64: {f64['iters']} iterations in {f64['wall']:.3f} s")
     value = f64["vox_iters"] / f64["wall"]
 
-    # roofline of the dominant kernel, from the HIP-event times of the timed steps
+    # roofline of the dominant kernel, from the HIP-event times of the same steps
     kt = f64["ktimes"]
     loop_kernels = ("nr_update", "fft_z_r2c", "fft_y_fwd", "fft_x_green", "fft_y_inv", "fft_z_c2r")
     dom = max(loop_kernels, key=lambda k: kt.get(k, (0.0, 0))[0])
@@ -214,6 +240,8 @@ def main():
         "nr_steps_per_voxel_iter": nr_steps_total / f64["vox_iters"],
         "nr_steps_max": max(i["max_nr_steps"] for i in f64["infos"]),
         "ms_prologue": sum(i["ms_prologue"] for i in f64["infos"]) / args.steps,
+        "ms_per_step_host_launched": 1e3 * f64["profiled_wall"] / args.steps,
+        "kernel_pass_same_bits": f64["profiled_same_bits"],
         "roofline": roof,
         "kernels": per_kernel,
         "mixed": mixed,

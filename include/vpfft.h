@@ -120,7 +120,10 @@ int vpfft_grid_download(void* grid, int field, double* host);
  * edot: macroscopic strain rate (EigenRep); avg_stress (out, may be NULL):
  * the returned <sigma>; info (out, may be NULL).  A step that reaches
  * max_iter without meeting tol returns VPFFT_OK with info->converged = 0
- * (the reference is silent about it). */
+ * (the reference is silent about it).  The fixed-point loop runs as a HIP
+ * graph of VPFFT_GRAPH_ITERS iterations (environment, read at grid creation;
+ * default 4, 0 = launch every kernel from the host) replayed until the
+ * device-side convergence test stops it; both forms give the same bits. */
 int vpfft_run_strain_step(void* grid, const double edot[5], int max_iter, double dt, double tol,
                           double avg_stress[5], vpfft_step_info* info);
 
@@ -170,7 +173,8 @@ int vpfft_generate_voronoi(int nx, int ny, int nz, int num_grains, uint64_t seed
 /* Per-kernel device time (no reference counterpart; the reference has no
  * timers, README.make:40-43).  When enabled, every kernel launch of a strain
  * step is bracketed by HIP events on the grid's stream and its elapsed time
- * summed per kernel.  Enabling resets the totals. */
+ * summed per kernel.  Enabling resets the totals and launches the loop's
+ * kernels from the host instead of replaying a graph (same results). */
 int vpfft_grid_set_profiling(void* grid, int enable);
 /* Copies up to `capacity` totals (ms) and launch counts; returns the number of kernels. */
 int vpfft_grid_kernel_times(void* grid, double* total_ms, long* launches, int capacity);

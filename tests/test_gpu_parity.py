@@ -237,3 +237,28 @@ def test_step_is_deterministic(gpu, vpfft):
             outs.append((a, g.Stress()))
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("graph_iters", ["3", "4"])
+def test_graph_loop_matches_host_launched_loop(gpu, vpfft, monkeypatch, graph_iters):
+    """The fixed-point loop replayed as a captured graph of several iterations
+    (kernels return at entry once the device loop state says done) gives the
+    bits of the loop launched kernel by kernel from the host, including the
+    iteration count when max_iter is not a multiple of the graph length."""
+    dims = (24, 20, 16)
+    R, _ = polycrystal(None, 24, 9, dims)
+    runs = []
+    for mode in ("0", graph_iters):
+        monkeypatch.setenv("VPFFT_GRAPH_ITERS", mode)
+        with vpfft.MaterialGrid(*dims, vpfft.Material.uniform(1.0, 10)) as g:
+            g.SendDataAndInitialize(R)
+            out = []
+            for max_iter, tol in ((7, 1e-12), (100, 1e-6), (100, 1e-6)):
+                a = g.RunSingleStrainStep(edot_tension(), max_iter, 0.005, tol)
+                out.append((a, g.last_info.iterations, g.Stress(), g.Orientation(), g.CRSS()))
+            runs.append(out)
+    assert runs[0][0][1] == 7
+    for s0, s1 in zip(runs[0], runs[1]):
+        assert s0[1] == s1[1]
+        for x0, x1 in zip(s0[:1] + s0[2:], s1[:1] + s1[2:]):
+            assert np.array_equal(x0, x1)

@@ -98,6 +98,19 @@ struct Grid {
   // hardening law whose matrix rows and Voce constants are slip-independent)
   bool crss_uniform = false;
   bool hardening_uniform = false;
+  // fixed-point loop: device loop state and a captured graph of several
+  // iterations (VPFFT_GRAPH_ITERS, 0 = launch every kernel from the host)
+  LoopState* dls = nullptr;
+  LoopState* hls = nullptr;
+  int graph_iters = 4;
+  hipGraphExec_t gexec = nullptr;
+  struct GraphKey {
+    int prec, max_iter, waves, uniform;
+    double tol;
+    bool operator==(const GraphKey& o) const {
+      return prec == o.prec && max_iter == o.max_iter && waves == o.waves && uniform == o.uniform && tol == o.tol;
+    }
+  } gkey{-1, 0, 0, 0, 0.0};
   int phi_prec = 0, dtil_prec = 0;  // precision the field buffers currently hold
   // per-kernel timing with HIP events on the grid's stream
   bool profile = false;
@@ -138,6 +151,7 @@ static void grid_free(Grid* g) {
   for (auto& pr : g->kev)
     for (auto& e : pr)
       if (e) (void)hipEventDestroy(e);
+  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
   if (g->st) (void)hipStreamDestroy(g->st);
   delete g;
 }
@@ -238,23 +252,23 @@ static void kcollect(Grid* g) {
   }
 }
 
-static int run_fft_green(Grid* g, int prec) {
+static int run_fft_green(Grid* g, int prec, const int* done) {
   void* const* tw = prec == 0 ? g->tw64 : g->tw32;
   const void* gc = prec == 0 ? (const void*)g->dG64 : (const void*)g->dG32;
   kbegin(g, K_Z_R2C);
-  HIP_TRY(launch_z_r2c(prec, g->phi, g->spec, g->geom, g->plan[2], tw[2], g->st));
+  HIP_TRY(launch_z_r2c(prec, g->phi, g->spec, g->geom, g->plan[2], tw[2], done, g->st));
   kend(g, K_Z_R2C);
   kbegin(g, K_Y_FWD);
-  HIP_TRY(launch_y(prec, g->spec, g->geom, g->plan[1], tw[1], -1, g->st));
+  HIP_TRY(launch_y(prec, g->spec, g->geom, g->plan[1], tw[1], -1, done, g->st));
   kend(g, K_Y_FWD);
   kbegin(g, K_X_GREEN);
-  HIP_TRY(launch_x_green(prec, g->spec, g->geom, g->plan[0], tw[0], gc, g->st));
+  HIP_TRY(launch_x_green(prec, g->spec, g->geom, g->plan[0], tw[0], gc, done, g->st));
   kend(g, K_X_GREEN);
   kbegin(g, K_Y_INV);
-  HIP_TRY(launch_y(prec, g->spec, g->geom, g->plan[1], tw[1], +1, g->st));
+  HIP_TRY(launch_y(prec, g->spec, g->geom, g->plan[1], tw[1], +1, done, g->st));
   kend(g, K_Y_INV);
   kbegin(g, K_Z_C2R);
-  HIP_TRY(launch_z_c2r(prec, g->spec, g->dtil, g->geom, g->plan[2], tw[2], 1.0 / (double)g->n, g->st));
+  HIP_TRY(launch_z_c2r(prec, g->spec, g->dtil, g->geom, g->plan[2], tw[2], 1.0 / (double)g->n, done, g->st));
   kend(g, K_Z_C2R);
   g->dtil_prec = prec;
   return VPFFT_OK;
@@ -350,6 +364,7 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
   g->prec = d->precision;
   g->mat = *mat;
   if (const char* w = std::getenv("VPFFT_NR_WAVES")) g->nr_waves = std::atoi(w) >= 2 ? 2 : 1;
+  if (const char* k = std::getenv("VPFFT_GRAPH_ITERS")) g->graph_iters = std::max(0, std::min(64, std::atoi(k)));
   g->hardening_uniform = true;
   for (int i = 0; i < kSlips; i++) {
     if (mat->tau0[i] != mat->tau0[0] || mat->tau1[i] != mat->tau1[0] || mat->theta0[i] != mat->theta0[0] ||
@@ -415,6 +430,7 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
   ALLOC(g->dG32, sizeof(GreenConst<float>));
   ALLOC(g->partial, sizeof(double) * std::max<long>(4096L * 18, nr_partial_rows(n) * kPartialStride));
   ALLOC(g->dprol, sizeof(PrologueRecord));
+  ALLOC(g->dls, sizeof(LoopState));
   g->staging_elems = std::min<long>(60L * n, 8L << 20);
   ALLOC(g->staging, sizeof(double) * g->staging_elems);
 #undef ALLOC
@@ -422,10 +438,30 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
   g->hC = (StepConst*)p;
   if ((rc = halloc(g, &p, sizeof(PrologueRecord)))) { grid_free(g); return rc; }
   g->hprol = (PrologueRecord*)p;
+  if ((rc = halloc(g, &p, sizeof(LoopState)))) { grid_free(g); return rc; }
+  g->hls = (LoopState*)p;
   if ((rc = ensure_records(g, 128))) { grid_free(g); return rc; }
   for (int a = 0; a < 3; a++)
     if ((rc = make_twiddles(g, a, dims[a]))) { grid_free(g); return rc; }
   *out = g;
+  return VPFFT_OK;
+}
+
+// One fixed-point iteration: FFT + Green + inverse FFT, the Newton kernel,
+// and the device-side convergence test.  Every kernel returns at entry once
+// the loop state says done, so the same sequence can be captured in a graph.
+static int enqueue_iteration(Grid* g, int prec, double tol, int max_iter) {
+  const int* done = &g->dls->done;
+  int rc;
+  if ((rc = run_fft_green(g, prec, done))) return rc;
+  kbegin(g, K_NR);
+  HIP_TRY(launch_nr(prec, g->nr_waves, g->crss_uniform, g->n, g->dtil, g->phi, g->sigma, g->lambda, g->R, g->crss,
+                    g->dC, g->partial, done, g->st));
+  kend(g, K_NR);
+  g->phi_prec = prec;
+  kbegin(g, K_ITER_FINALIZE);
+  HIP_TRY(launch_iter_finalize(g->n, g->partial, g->dC, tol, max_iter, g->dls, g->drec, g->st));
+  kend(g, K_ITER_FINALIZE);
   return VPFFT_OK;
 }
 
@@ -464,28 +500,58 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
     return set_error(VPFFT_E_DEGENERATE, "%.0f voxels with max resolved shear stress < 1e-10 (initialization)",
                      g->hprol->degenerate);
   // fixed-point loop: MaterialGrid.cpp:930-1068
+  HIP_TRY(launch_loop_reset(g->dls, g->st));
   int it = 0;
   double nr_total = 0;
-  for (;;) {
-    if ((rc = run_fft_green(g, prec))) return rc;
-    kbegin(g, K_NR);
-    HIP_TRY(launch_nr(prec, g->nr_waves, g->crss_uniform, n, g->dtil, g->phi, g->sigma, g->lambda, g->R, g->crss,
-                      g->dC, g->partial, g->st));
-    kend(g, K_NR);
-    g->phi_prec = prec;
-    kbegin(g, K_ITER_FINALIZE);
-    HIP_TRY(launch_iter_finalize(n, nb, g->partial, g->dC, tol, max_iter, it, g->drec, g->st));
-    kend(g, K_ITER_FINALIZE);
-    HIP_TRY(hipMemcpyAsync(&g->hrec[it], &g->drec[it], sizeof(IterRecord), hipMemcpyDeviceToHost, g->st));
+  const bool use_graph = g->graph_iters > 0 && !g->profile;
+  if (use_graph) {
+    const Grid::GraphKey key{prec, max_iter, g->nr_waves, g->crss_uniform ? 1 : 0, tol};
+    if (!g->gexec || !(g->gkey == key)) {
+      if (g->gexec) {
+        HIP_TRY(hipGraphExecDestroy(g->gexec));
+        g->gexec = nullptr;
+      }
+      hipGraph_t graph;
+      HIP_TRY(hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal));
+      int crc = VPFFT_OK;
+      for (int k = 0; k < g->graph_iters && crc == VPFFT_OK; k++) crc = enqueue_iteration(g, prec, tol, max_iter);
+      hipError_t ce = hipStreamEndCapture(g->st, &graph);
+      if (crc) return crc;
+      HIP_TRY(ce);
+      hipError_t ie = hipGraphInstantiate(&g->gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIP_TRY(ie);
+      g->gkey = key;
+    }
+    for (;;) {
+      HIP_TRY(hipGraphLaunch(g->gexec, g->st));
+      HIP_TRY(hipMemcpyAsync(g->hls, g->dls, sizeof(LoopState), hipMemcpyDeviceToHost, g->st));
+      HIP_TRY(hipStreamSynchronize(g->st));
+      if (g->hls->done) break;
+    }
+    it = g->hls->iter;
+    HIP_TRY(hipMemcpyAsync(g->hrec, g->drec, sizeof(IterRecord) * it, hipMemcpyDeviceToHost, g->st));
     HIP_TRY(hipStreamSynchronize(g->st));
-    kcollect(g);
-    const IterRecord& r = g->hrec[it];
-    nr_total += r.nr_steps;
-    if (r.degenerate > 0)
-      return set_error(VPFFT_E_DEGENERATE, "%.0f voxels with max resolved shear stress < 1e-10 (iteration %d)",
-                       r.degenerate, it + 1);
-    it++;
-    if (r.done) break;
+    for (int i = 0; i < it; i++) {
+      nr_total += g->hrec[i].nr_steps;
+      if (g->hrec[i].degenerate > 0)
+        return set_error(VPFFT_E_DEGENERATE, "%.0f voxels with max resolved shear stress < 1e-10 (iteration %d)",
+                         g->hrec[i].degenerate, i + 1);
+    }
+  } else {
+    for (;;) {
+      if ((rc = enqueue_iteration(g, prec, tol, max_iter))) return rc;
+      HIP_TRY(hipMemcpyAsync(&g->hrec[it], &g->drec[it], sizeof(IterRecord), hipMemcpyDeviceToHost, g->st));
+      HIP_TRY(hipStreamSynchronize(g->st));
+      kcollect(g);
+      const IterRecord& r = g->hrec[it];
+      nr_total += r.nr_steps;
+      if (r.degenerate > 0)
+        return set_error(VPFFT_E_DEGENERATE, "%.0f voxels with max resolved shear stress < 1e-10 (iteration %d)",
+                         r.degenerate, it + 1);
+      it++;
+      if (r.done) break;
+    }
   }
   HIP_TRY(hipEventRecord(g->ev[2], g->st));
   const IterRecord& last = g->hrec[it - 1];
@@ -749,7 +815,7 @@ int vpfft_strain_rate_evolution(void* grid, const double L0[25], const double* p
   int rc = vpfft::upload_field5(g, phi_in, g->phi, g->prec);
   if (rc) return rc;
   g->phi_prec = g->prec;
-  if ((rc = vpfft::run_fft_green(g, g->prec))) return rc;
+  if ((rc = vpfft::run_fft_green(g, g->prec, nullptr))) return rc;
   e = hipStreamSynchronize(g->st);
   if (e != hipSuccess) return set_error(VPFFT_E_DEVICE, "%s", hipGetErrorString(e));
   return vpfft::download_field5(g, g->dtil, dtilde_out, g->dtil_prec);

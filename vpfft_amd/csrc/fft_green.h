@@ -34,14 +34,16 @@ struct GreenConst {
 // Factor n into radices {8,4,2,3,5,7}; returns false for other prime factors.
 bool make_axis_plan(int n, AxisPlan* ap);
 
-// prec: 0 = FP64 storage/compute, 1 = FP32
+// prec: 0 = FP64 storage/compute, 1 = FP32.  done: optional device flag; a
+// kernel whose flag is set returns at entry (graph-captured iterations after
+// convergence do nothing).
 hipError_t launch_z_r2c(int prec, const void* f, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
-                        hipStream_t st);
+                        const int* done, hipStream_t st);
 hipError_t launch_z_c2r(int prec, const void* s, void* f, const FftGeom& g, const AxisPlan& ap, const void* tw,
-                        double scale, hipStream_t st);
+                        double scale, const int* done, hipStream_t st);
 hipError_t launch_y(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int sign,
-                    hipStream_t st);
+                    const int* done, hipStream_t st);
 hipError_t launch_x_green(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, const void* gc,
-                          hipStream_t st);
+                          const int* done, hipStream_t st);
 
 }  // namespace vpfft

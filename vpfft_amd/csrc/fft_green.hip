@@ -197,8 +197,10 @@ __device__ typename C2<T>::t* lds_fft(typename C2<T>::t* buf0, typename C2<T>::t
 // ---------------------------------------------------------------------------
 template <typename T, int B>
 __global__ void __launch_bounds__(256) k_z_r2c(const T* __restrict__ f, typename C2<T>::t* __restrict__ s,
-                                               FftGeom g, AxisPlan ap, const typename C2<T>::t* __restrict__ tw) {
+                                               FftGeom g, AxisPlan ap, const typename C2<T>::t* __restrict__ tw,
+                                               const int* __restrict__ done) {
   using C = typename C2<T>::t;
+  if (done && *done) return;
   constexpr int P = 3 * B;
   constexpr int LS = P;
   extern __shared__ unsigned char smem_raw[];
@@ -247,8 +249,9 @@ __global__ void __launch_bounds__(256) k_z_r2c(const T* __restrict__ f, typename
 template <typename T, int B>
 __global__ void __launch_bounds__(256) k_z_c2r(const typename C2<T>::t* __restrict__ s, T* __restrict__ f,
                                                FftGeom g, AxisPlan ap, const typename C2<T>::t* __restrict__ tw,
-                                               T scale) {
+                                               T scale, const int* __restrict__ done) {
   using C = typename C2<T>::t;
+  if (done && *done) return;
   constexpr int P = 3 * B;
   constexpr int LS = P;
   extern __shared__ unsigned char smem_raw[];
@@ -297,8 +300,10 @@ __global__ void __launch_bounds__(256) k_z_c2r(const typename C2<T>::t* __restri
 // ---------------------------------------------------------------------------
 template <typename T, int B>
 __global__ void __launch_bounds__(256) k_y(typename C2<T>::t* __restrict__ s, FftGeom g, AxisPlan ap,
-                                           const typename C2<T>::t* __restrict__ tw, int sign) {
+                                           const typename C2<T>::t* __restrict__ tw, int sign,
+                                           const int* __restrict__ done) {
   using C = typename C2<T>::t;
+  if (done && *done) return;
   extern __shared__ unsigned char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
   C* buf1 = buf0 + B * g.ny;
@@ -426,8 +431,9 @@ __device__ __forceinline__ void green_apply(typename C2<T>::t ph[5], int ix, int
 template <typename T, int B, int E>
 __global__ void __launch_bounds__(512) k_x_green(typename C2<T>::t* __restrict__ s, FftGeom g, AxisPlan ap,
                                                  const typename C2<T>::t* __restrict__ tw,
-                                                 const GreenConst<T>* __restrict__ gc) {
+                                                 const GreenConst<T>* __restrict__ gc, const int* __restrict__ done) {
   using C = typename C2<T>::t;
+  if (done && *done) return;
   extern __shared__ unsigned char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
   C* buf1 = buf0 + B * g.nx;
@@ -521,13 +527,13 @@ static int pick_run(int n, int budget) {
 
 template <typename T>
 static hipError_t launch_z_r2c_t(const T* f, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
-                                 hipStream_t st) {
+                                 const int* done, hipStream_t st) {
   using C = typename C2<T>::t;
   const int B = pick_rows(g.nz, sizeof(T) == 8 ? 1536 : 3072);
   const unsigned grid = (unsigned)((g.nrows + B - 1) / B);
   const size_t lds = 2 * sizeof(C) * 3 * B * g.nz;
   switch (B) {
-#define CASE(BB) case BB: hipLaunchKernelGGL((k_z_r2c<T, BB>), dim3(grid), dim3(256), lds, st, f, (C*)s, g, ap, (const C*)tw); break;
+#define CASE(BB) case BB: hipLaunchKernelGGL((k_z_r2c<T, BB>), dim3(grid), dim3(256), lds, st, f, (C*)s, g, ap, (const C*)tw, done); break;
     CASE(1) CASE(2) CASE(4) CASE(8) default: CASE(16)
 #undef CASE
   }
@@ -536,13 +542,13 @@ static hipError_t launch_z_r2c_t(const T* f, void* s, const FftGeom& g, const Ax
 
 template <typename T>
 static hipError_t launch_z_c2r_t(const void* s, T* f, const FftGeom& g, const AxisPlan& ap, const void* tw,
-                                 double scale, hipStream_t st) {
+                                 double scale, const int* done, hipStream_t st) {
   using C = typename C2<T>::t;
   const int B = pick_rows(g.nz, sizeof(T) == 8 ? 1536 : 3072);
   const unsigned grid = (unsigned)((g.nrows + B - 1) / B);
   const size_t lds = 2 * sizeof(C) * 3 * B * g.nz;
   switch (B) {
-#define CASE(BB) case BB: hipLaunchKernelGGL((k_z_c2r<T, BB>), dim3(grid), dim3(256), lds, st, (const C*)s, f, g, ap, (const C*)tw, (T)scale); break;
+#define CASE(BB) case BB: hipLaunchKernelGGL((k_z_c2r<T, BB>), dim3(grid), dim3(256), lds, st, (const C*)s, f, g, ap, (const C*)tw, (T)scale, done); break;
     CASE(1) CASE(2) CASE(4) CASE(8) default: CASE(16)
 #undef CASE
   }
@@ -551,14 +557,14 @@ static hipError_t launch_z_c2r_t(const void* s, T* f, const FftGeom& g, const Ax
 
 template <typename T>
 static hipError_t launch_y_t(void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int sign,
-                             hipStream_t st) {
+                             const int* done, hipStream_t st) {
   using C = typename C2<T>::t;
   const int B = pick_run(g.ny, sizeof(T) == 8 ? 2048 : 4096);
   const int runs = (g.nzh + B - 1) / B;
   const unsigned grid = (unsigned)(5L * g.nx * runs);
   const size_t lds = 2 * sizeof(C) * B * g.ny;
   switch (B) {
-#define CASE(BB) case BB: hipLaunchKernelGGL((k_y<T, BB>), dim3(grid), dim3(256), lds, st, (C*)s, g, ap, (const C*)tw, sign); break;
+#define CASE(BB) case BB: hipLaunchKernelGGL((k_y<T, BB>), dim3(grid), dim3(256), lds, st, (C*)s, g, ap, (const C*)tw, sign, done); break;
     CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) default: CASE(64)
 #undef CASE
   }
@@ -567,7 +573,7 @@ static hipError_t launch_y_t(void* s, const FftGeom& g, const AxisPlan& ap, cons
 
 template <typename T>
 static hipError_t launch_x_green_t(void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
-                                   const void* gc, hipStream_t st) {
+                                   const void* gc, const int* done, hipStream_t st) {
   using C = typename C2<T>::t;
   // runs of at least 128 bytes along kz (8 complex doubles, 16 complex
   // floats); 512 threads keep 5 * E complex values per thread in registers
@@ -580,7 +586,7 @@ static hipError_t launch_x_green_t(void* s, const FftGeom& g, const AxisPlan& ap
   const int cnt = B * g.nx;
   const int E = (cnt + 511) / 512;
   const GreenConst<T>* G = (const GreenConst<T>*)gc;
-#define LAUNCH(BB, EE) hipLaunchKernelGGL((k_x_green<T, BB, EE>), dim3(grid), dim3(512), lds, st, (C*)s, g, ap, (const C*)tw, G)
+#define LAUNCH(BB, EE) hipLaunchKernelGGL((k_x_green<T, BB, EE>), dim3(grid), dim3(512), lds, st, (C*)s, g, ap, (const C*)tw, G, done)
 #define EC(BB)                         \
   case BB:                             \
     if (E <= 1) LAUNCH(BB, 1);         \
@@ -597,22 +603,23 @@ static hipError_t launch_x_green_t(void* s, const FftGeom& g, const AxisPlan& ap
 }
 
 hipError_t launch_z_r2c(int prec, const void* f, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
-                        hipStream_t st) {
-  return prec == 0 ? launch_z_r2c_t<double>((const double*)f, s, g, ap, tw, st)
-                   : launch_z_r2c_t<float>((const float*)f, s, g, ap, tw, st);
+                        const int* done, hipStream_t st) {
+  return prec == 0 ? launch_z_r2c_t<double>((const double*)f, s, g, ap, tw, done, st)
+                   : launch_z_r2c_t<float>((const float*)f, s, g, ap, tw, done, st);
 }
 hipError_t launch_z_c2r(int prec, const void* s, void* f, const FftGeom& g, const AxisPlan& ap, const void* tw,
-                        double scale, hipStream_t st) {
-  return prec == 0 ? launch_z_c2r_t<double>(s, (double*)f, g, ap, tw, scale, st)
-                   : launch_z_c2r_t<float>(s, (float*)f, g, ap, tw, scale, st);
+                        double scale, const int* done, hipStream_t st) {
+  return prec == 0 ? launch_z_c2r_t<double>(s, (double*)f, g, ap, tw, scale, done, st)
+                   : launch_z_c2r_t<float>(s, (float*)f, g, ap, tw, scale, done, st);
 }
 hipError_t launch_y(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int sign,
-                    hipStream_t st) {
-  return prec == 0 ? launch_y_t<double>(s, g, ap, tw, sign, st) : launch_y_t<float>(s, g, ap, tw, sign, st);
+                    const int* done, hipStream_t st) {
+  return prec == 0 ? launch_y_t<double>(s, g, ap, tw, sign, done, st) : launch_y_t<float>(s, g, ap, tw, sign, done, st);
 }
 hipError_t launch_x_green(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, const void* gc,
-                          hipStream_t st) {
-  return prec == 0 ? launch_x_green_t<double>(s, g, ap, tw, gc, st) : launch_x_green_t<float>(s, g, ap, tw, gc, st);
+                          const int* done, hipStream_t st) {
+  return prec == 0 ? launch_x_green_t<double>(s, g, ap, tw, gc, done, st)
+                   : launch_x_green_t<float>(s, g, ap, tw, gc, done, st);
 }
 
 }  // namespace vpfft

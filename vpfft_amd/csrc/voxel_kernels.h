@@ -38,6 +38,11 @@ struct IterRecord {
   int iteration;
 };
 
+struct LoopState {
+  int iter;  // iterations completed in this strain step
+  int done;  // set by the convergence test (MaterialGrid.cpp:1066-1068)
+};
+
 struct PrologueRecord {
   double L0[25];
   double degenerate;
@@ -54,9 +59,10 @@ hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenCon
 // uniform_tau: every slip system of a voxel has the same CRSS (crss[0][v] is read)
 hipError_t launch_nr(int prec, int waves, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
                      double* lambda, const double* R, const double* crss, const StepConst* C, double* partial,
-                     hipStream_t st);
-hipError_t launch_iter_finalize(long n, int nblocks, const double* partial, const StepConst* C, double tol,
-                                int max_iter, int it, IterRecord* rec, hipStream_t st);
+                     const int* done, hipStream_t st);
+hipError_t launch_iter_finalize(long n, const double* partial, const StepConst* C, double tol,
+                                int max_iter, LoopState* ls, IterRecord* rec, hipStream_t st);
+hipError_t launch_loop_reset(LoopState* ls, hipStream_t st);
 hipError_t launch_epilogue(long n, double* R, double* crss, double* acc, const double* sigma, const StepConst* C,
                            hipStream_t st);
 hipError_t launch_aos_to_soa(const double* aos, double* soa, long count, int width, long soa_stride, long offset,

@@ -222,6 +222,18 @@ __global__ void __launch_bounds__(256) k_L0_finalize(long n, int nblocks, const 
   }
 }
 
+__global__ void k_loop_reset(LoopState* ls) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ls->iter = 0;
+    ls->done = 0;
+  }
+}
+
+hipError_t launch_loop_reset(LoopState* ls, hipStream_t st) {
+  hipLaunchKernelGGL(k_loop_reset, dim3(1), dim3(64), 0, st, ls);
+  return hipGetLastError();
+}
+
 __global__ void k_derive(StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32) {
   if (threadIdx.x == 0 && blockIdx.x == 0) derive_constants(C, g64, g32);
 }
@@ -238,7 +250,8 @@ k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
                                                       double* __restrict__ lambda, const double* __restrict__ R,
                                                       const double* __restrict__ crss,
                                                       const StepConst* __restrict__ Cp,
-                                                      double* __restrict__ partial) {
+                                                      double* __restrict__ partial, const int* __restrict__ done) {
+  if (done && *done) return;
   const StepConst& C = *Cp;
   double acc[14];
 #pragma unroll
@@ -318,7 +331,8 @@ k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
 // Reduce rows [256 b, 256 b + 256) of the per-block partials into row b of
 // out: 14 sums then 2 maxima, fixed tree order.
 __global__ void __launch_bounds__(256) k_fold_partials(const double* __restrict__ in, int rows,
-                                                       double* __restrict__ out) {
+                                                       double* __restrict__ out, const int* __restrict__ done) {
+  if (done && *done) return;
   __shared__ double sh[256][16];
   const long r = (long)blockIdx.x * 256 + threadIdx.x;
   for (int w = 0; w < 16; w++) sh[threadIdx.x][w] = r < rows ? in[r * kPartialStride + w] : 0.0;
@@ -334,9 +348,13 @@ __global__ void __launch_bounds__(256) k_fold_partials(const double* __restrict_
     for (int w = 0; w < 16; w++) out[(long)blockIdx.x * kPartialStride + w] = sh[0][w];
 }
 
+// LoopState: the iteration counter and the done flag, both on the device so
+// a captured graph of several iterations needs no host round trip.
 __global__ void __launch_bounds__(256) k_iter_finalize(long n, int nblocks, const double* __restrict__ partial,
                                                        const StepConst* __restrict__ C, double tol, int max_iter,
-                                                       int it, IterRecord* rec) {
+                                                       LoopState* ls, IterRecord* rec) {
+  if (ls->done) return;
+  const int it = ls->iter;
   __shared__ double sh[256][16];
   double acc[16];
   for (int w = 0; w < 16; w++) acc[w] = 0.0;
@@ -376,7 +394,9 @@ __global__ void __launch_bounds__(256) k_iter_finalize(long n, int nblocks, cons
     R.max_nr_steps = sh[0][15];
     R.iteration = it + 1;
     // loop condition, MaterialGrid.cpp:1066-1068
-    R.done = !((it + 1) < max_iter && (es > tol || ee > tol));
+    R.done = R.degenerate > 0 || !((it + 1) < max_iter && (es > tol || ee > tol));
+    ls->iter = it + 1;
+    ls->done = R.done;
   }
 }
 
@@ -541,11 +561,11 @@ hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenCon
 
 hipError_t launch_nr(int prec, int waves, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
                      double* lambda, const double* R, const double* crss, const StepConst* C, double* partial,
-                     hipStream_t st) {
+                     const int* done, hipStream_t st) {
   const int nb = nr_blocks(n);
 #define NR_LAUNCH(TT, WW, UU)                                                                                    \
   hipLaunchKernelGGL((k_nr<TT, WW, UU>), dim3(nb), dim3(kPointThreads), 0, st, n, (const TT*)dtil, (TT*)phi, sigma, \
-                     lambda, R, crss, C, partial)
+                     lambda, R, crss, C, partial, done)
 #define NR_W(TT, UU)                   \
   if (waves >= 2) NR_LAUNCH(TT, 2, UU); \
   else NR_LAUNCH(TT, 1, UU);
@@ -560,18 +580,18 @@ hipError_t launch_nr(int prec, int waves, bool uniform_tau, long n, const void* 
   if (e != hipSuccess) return e;
   // fold the per-block partials in a fixed order: nb -> ceil(nb / 256) rows
   const int nb2 = (nb + 255) / 256;
-  hipLaunchKernelGGL(k_fold_partials, dim3(nb2), dim3(256), 0, st, partial, nb, partial + (long)nb * kPartialStride);
+  hipLaunchKernelGGL(k_fold_partials, dim3(nb2), dim3(256), 0, st, partial, nb, partial + (long)nb * kPartialStride,
+                     done);
   return hipGetLastError();
 }
 
-hipError_t launch_iter_finalize(long n, int nblocks, const double* partial, const StepConst* C, double tol,
-                                int max_iter, int it, IterRecord* rec, hipStream_t st) {
+hipError_t launch_iter_finalize(long n, const double* partial, const StepConst* C, double tol,
+                                int max_iter, LoopState* ls, IterRecord* rec, hipStream_t st) {
   // launch_nr left ceil(nr_blocks / 256) folded rows after the per-block partials
   const int nb = nr_blocks(n);
   const int nb2 = (nb + 255) / 256;
-  (void)nblocks;
   hipLaunchKernelGGL(k_iter_finalize, dim3(1), dim3(256), 0, st, n, nb2, partial + (long)nb * kPartialStride, C,
-                     tol, max_iter, it, rec);
+                     tol, max_iter, ls, rec);
   return hipGetLastError();
 }
 
