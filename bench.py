@@ -31,7 +31,13 @@ FP64_PEAK_TFLOPS = 78.6
 # derivation is in DESIGN.md, "Roofline accounting"):
 NR_FLOPS_FIXED = 1500.0       # per voxel per iteration, outside the Newton steps
 NR_FLOPS_PER_STEP = 1150.0    # per Newton step
-NR_BYTES = 408.0              # d~, sigma, lambda, R, crss read; sigma, lambda, phi written (FP64)
+# d~, sigma, lambda (40 B each), R (72 B) and one CRSS (8 B) read; sigma,
+# lambda, phi written (40 B each): the uniform-CRSS variant the workload runs
+# (every voxel's CRSS equal and kept equal by the hardening law)
+NR_BYTES = 320.0
+# measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a
+# plain copy of known size) for the default workload
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "strain_step_256cubed_hbm_traffic.md")
 
 
 def log(*a):
@@ -43,6 +49,17 @@ def fft_bytes(kernel, n, nc, elt):
     if kernel in ("fft_z_r2c", "fft_z_c2r"):
         return 5 * n * elt + 5 * nc * 2 * elt
     return 2 * 5 * nc * 2 * elt  # y / x passes: read + write the half spectrum
+
+
+def measured_traffic(args, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter profile, when
+    the run is the workload that profile measured (else None)."""
+    if (args.size, args.grains, args.nexp, args.seed) != (256, 4096, 20, 0) or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    import re
+    m = re.search(r"```json\n(.*?)\n```", open(TRAFFIC_PROFILE).read(), re.S)
+    e = json.loads(m.group(1)).get(kernel) if m else None
+    return None if e is None else e["read_bytes"] + e["write_bytes"]
 
 
 def run_form(V, R, mat, edot, args, precision, steps, warmup, profile):
@@ -135,8 +152,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-max-iters", type=int, default=20)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="first copy this many MiB device to device (torch) three times: a kernel of known "
-                         "traffic that scales FETCH_SIZE / WRITE_SIZE counters read under rocprofv3")
+                    help="only copy this many MiB device to device (torch) three times and exit: a kernel of "
+                         "known traffic that calibrates FETCH_SIZE / WRITE_SIZE counters read under rocprofv3")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -144,9 +161,6 @@ def main():
         print("bench.py runs on one GPU: the whole voxel grid lives in one MI355X's HBM and there is no "
               "multi-GPU decomposition in this build (--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
-
-    import vpfft_amd as V
-    V.load_library()
 
     if args.copy_probe_mib > 0:
         import torch
@@ -156,7 +170,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         log(f"[bench] copy probe: 3 x {args.copy_probe_mib} MiB read + written")
-        del a, b
+        return
+
+    import vpfft_amd as V
+    V.load_library()
 
     log(f"[bench] generating {args.size}^3 microstructure, {args.grains} grains")
     R, _, _ = V.generate_voronoi(args.size, args.size, args.size, args.grains, args.seed)
@@ -193,12 +210,12 @@ def main():
     if dom == "nr_update":
         ach = per_kernel[dom]["TFLOPs"]
         roof = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "nr_update",
+                "frac": ach / FP64_PEAK_TFLOPS, "traffic": measured_traffic(args, dom), "kernel": "nr_update",
                 "note": "FP64 vector ALU (no MFMA): MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s"}
     else:
         ach = per_kernel[dom]["GBps"]
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom}
+                "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(args, dom), "kernel": dom}
 
     mixed = None
     if not args.no_mixed:
@@ -239,6 +256,7 @@ def main():
         "loop_only_voxel_iters_per_s": f64["vox_iters"] / (loop_ms / 1e3),
         "nr_steps_per_voxel_iter": nr_steps_total / f64["vox_iters"],
         "nr_steps_max": max(i["max_nr_steps"] for i in f64["infos"]),
+        "nr_lane_efficiency": nr_steps_total / sum(i["nr_wave_steps_total"] for i in f64["infos"]),
         "ms_prologue": sum(i["ms_prologue"] for i in f64["infos"]) / args.steps,
         "ms_per_step_host_launched": 1e3 * f64["profiled_wall"] / args.steps,
         "kernel_pass_same_bits": f64["profiled_same_bits"],

@@ -88,6 +88,9 @@ typedef struct {
   double nr_steps_total;                       /* ... summed over all iterations */
   double ms_prologue, ms_loop, ms_epilogue;    /* device time (HIP events) */
   double max_nr_steps;                         /* largest per-voxel Newton step count, last iteration */
+  double nr_wave_steps_total;                  /* sum over iterations and 64-voxel waves of 64 x the wave's
+                                                  largest step count: nr_steps_total / this = the Newton
+                                                  loop's lane efficiency */
 } vpfft_step_info;
 
 /* ---- grid lifetime: MaterialGrid(int,int,int) + InitializeMemory (MaterialGrid.cpp:312-394) */

@@ -84,6 +84,7 @@ class vpfft_step_info(C.Structure):
         ("ms_loop", C.c_double),
         ("ms_epilogue", C.c_double),
         ("max_nr_steps", C.c_double),
+        ("nr_wave_steps_total", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -92,6 +93,23 @@ class vpfft_step_info(C.Structure):
             v = getattr(self, name)
             d[name] = list(v) if hasattr(v, "__len__") else v
         return d
+
+
+class vpfft_deck(C.Structure):
+    """include/vpfft_io.h: Utilities::ConfigFile, Src/Config.h:21-52."""
+    _fields_ = [
+        ("num_x", C.c_int), ("num_y", C.c_int), ("num_z", C.c_int),
+        ("input_strain_rate", (C.c_double * 3) * 3),
+        ("max_nr_iter", C.c_int),
+        ("nr_tolerance", C.c_double),
+        ("time_step", C.c_double),
+        ("num_time_iter", C.c_int),
+        ("init_method", C.c_int),
+        ("microstructure_file", C.c_char * 1024),
+        ("crss", C.c_double * NSLIP),
+        ("gamma_dot_base", C.c_double * NSLIP),
+        ("rate_sensitivity", C.c_int * NSLIP),
+    ]
 
 
 _lib = None
@@ -104,6 +122,10 @@ EXPORTED = [
     "vpfft_run_vpfft", "vpfft_strain_rate_evolution", "vpfft_solve_constrained_batch",
     "vpfft_solve_taylor_batch", "vpfft_generate_voronoi", "vpfft_last_error", "vpfft_version",
     "vpfft_grid_set_profiling", "vpfft_grid_kernel_times", "vpfft_kernel_name",
+    # include/vpfft_io.h
+    "vpfft_deck_read", "vpfft_deck_parse", "vpfft_deck_material", "vpfft_deck_strain_rate",
+    "vpfft_two_grain_init", "vpfft_euler_deg_to_matrix", "vpfft_read_euler_file", "vpfft_write_euler_file",
+    "vpfft_write_material_file", "vpfft_read_material_file", "vpfft_run_deck",
 ]
 
 
@@ -138,6 +160,19 @@ def load_library(path: Optional[str] = None):
     lib.vpfft_grid_set_profiling.argtypes = [vp, C.c_int]
     lib.vpfft_grid_kernel_times.argtypes = [vp, dp, C.POINTER(C.c_long), C.c_int]
     lib.vpfft_kernel_name.argtypes = [C.c_int]
+    ip = C.POINTER(C.c_int32)
+    cp = C.c_char_p
+    lib.vpfft_deck_read.argtypes = [cp, C.POINTER(vpfft_deck)]
+    lib.vpfft_deck_parse.argtypes = [cp, C.POINTER(vpfft_deck)]
+    lib.vpfft_deck_material.argtypes = [C.POINTER(vpfft_deck), C.POINTER(vpfft_material)]
+    lib.vpfft_deck_strain_rate.argtypes = [C.POINTER(vpfft_deck), dp]
+    lib.vpfft_two_grain_init.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
+    lib.vpfft_euler_deg_to_matrix.argtypes = [dp, C.c_long, dp]
+    lib.vpfft_read_euler_file.argtypes = [cp, C.c_int, C.c_int, C.c_int, dp, dp, ip, ip]
+    lib.vpfft_write_euler_file.argtypes = [cp, C.c_int, C.c_int, C.c_int, dp, ip, ip]
+    lib.vpfft_write_material_file.argtypes = [cp, C.c_int, C.c_int, C.c_int, dp, dp]
+    lib.vpfft_read_material_file.argtypes = [cp, C.c_int, C.c_int, C.c_int, dp, dp]
+    lib.vpfft_run_deck.argtypes = [cp, C.c_int, cp, cp, C.POINTER(vpfft_step_info), C.c_int]
     lib.vpfft_kernel_name.restype = C.c_char_p
     lib.vpfft_last_error.restype = C.c_char_p
     lib.vpfft_version.restype = C.c_char_p

@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvpfft.so")
-SOURCES = ["capi.hip", "fft_green.hip", "voxel_kernels.hip", "microstructure.cpp"]
+RUNNER = os.path.join(HERE, "vpfft_run")
+SOURCES = ["capi.hip", "fft_green.hip", "voxel_kernels.hip", "microstructure.cpp", "deck_io.cpp"]
 ARCH = os.environ.get("VPFFT_OFFLOAD_ARCH", "gfx950")
 
 
@@ -37,7 +38,7 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(outdir, src + ".o")
         cmd = [hipcc()] + common + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
-            cmd = [hipcc(), "-O3", "-fPIC", "-std=c++17", "-x", "c++", "-c",
+            cmd = [hipcc(), "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-x", "c++", "-c",
                    os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -49,6 +50,13 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # the deck-driven command-line runner, linked against the library next to it
+    cmd = [hipcc(), "-O2", "-std=c++17", "-Wall", "-x", "c++", os.path.join(CSRC, "vpfft_run.cpp"),
+           "-o", RUNNER + ".tmp", "-L", HERE, "-lvpfft", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(RUNNER + ".tmp", RUNNER)
     return LIB
 
 

@@ -502,7 +502,7 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
   // fixed-point loop: MaterialGrid.cpp:930-1068
   HIP_TRY(launch_loop_reset(g->dls, g->st));
   int it = 0;
-  double nr_total = 0;
+  double nr_total = 0, wave_total = 0;
   const bool use_graph = g->graph_iters > 0 && !g->profile;
   if (use_graph) {
     const Grid::GraphKey key{prec, max_iter, g->nr_waves, g->crss_uniform ? 1 : 0, tol};
@@ -534,6 +534,7 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
     HIP_TRY(hipStreamSynchronize(g->st));
     for (int i = 0; i < it; i++) {
       nr_total += g->hrec[i].nr_steps;
+      wave_total += g->hrec[i].nr_wave_steps;
       if (g->hrec[i].degenerate > 0)
         return set_error(VPFFT_E_DEGENERATE, "%.0f voxels with max resolved shear stress < 1e-10 (iteration %d)",
                          g->hrec[i].degenerate, i + 1);
@@ -546,6 +547,7 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
       kcollect(g);
       const IterRecord& r = g->hrec[it];
       nr_total += r.nr_steps;
+      wave_total += r.nr_wave_steps;
       if (r.degenerate > 0)
         return set_error(VPFFT_E_DEGENERATE, "%.0f voxels with max resolved shear stress < 1e-10 (iteration %d)",
                          r.degenerate, it + 1);
@@ -592,6 +594,7 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
     info->max_nr_steps = last.max_nr_steps;
     info->nr_steps_last = last.nr_steps;
     info->nr_steps_total = nr_total;
+    info->nr_wave_steps_total = wave_total;
     float ms = 0;
     (void)hipEventElapsedTime(&ms, g->ev[0], g->ev[1]);
     info->ms_prologue = ms;
@@ -826,7 +829,7 @@ namespace {
 struct BatchBufs {
   std::vector<void*> ptrs;
   ~BatchBufs() {
-    for (void* p : ptrs) hipFree(p);
+    for (void* p : ptrs) (void)hipFree(p);
   }
   void* get(size_t bytes) {
     void* p = nullptr;

@@ -10,7 +10,7 @@ namespace vpfft {
 // Fixed launch shape of every pointwise kernel: the voxel->thread assignment
 // and therefore the order of every reduction is independent of timing.
 constexpr int kPointThreads = 256;
-constexpr int kPartialStride = 16;
+constexpr int kPartialStride = 24;  // 15 sums + 2 maxima per block, padded
 
 // the Newton kernel runs one voxel per lane
 inline int nr_blocks(long n) { return (int)((n + kPointThreads - 1) / kPointThreads); }
@@ -34,6 +34,7 @@ struct IterRecord {
   double nr_steps;      // total Newton steps taken over all voxels
   double degenerate;    // voxels whose max resolved shear stress was < 1e-10
   double max_nr_steps;  // largest Newton step count of any voxel
+  double nr_wave_steps; // sum over waves of 64 x the wave's largest step count
   int done;
   int iteration;
 };

@@ -253,9 +253,9 @@ k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
                                                       double* __restrict__ partial, const int* __restrict__ done) {
   if (done && *done) return;
   const StepConst& C = *Cp;
-  double acc[14];
+  double acc[15];
 #pragma unroll
-  for (int w = 0; w < 14; w++) acc[w] = 0.0;
+  for (int w = 0; w < 15; w++) acc[w] = 0.0;
   double mx[2] = {0.0, 0.0};  // max NR error, max NR steps
   // one voxel per lane: nothing but the voxel's own state is live
   const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -325,27 +325,33 @@ k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
     mx[0] = fmax(mx[0], err);
     mx[1] = fmax(mx[1], (double)steps);
   }
-  block_sum<14>(acc, mx, partial + (long)blockIdx.x * kPartialStride);
+  // lane efficiency of the Newton loop: a wave runs as many steps as its
+  // slowest lane
+  double wmax = mx[1];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
+  if ((threadIdx.x & 63) == 0) acc[14] = 64.0 * wmax;
+  block_sum<15>(acc, mx, partial + (long)blockIdx.x * kPartialStride);
 }
 
 // Reduce rows [256 b, 256 b + 256) of the per-block partials into row b of
-// out: 14 sums then 2 maxima, fixed tree order.
+// out: 15 sums then 2 maxima, fixed tree order.
 __global__ void __launch_bounds__(256) k_fold_partials(const double* __restrict__ in, int rows,
                                                        double* __restrict__ out, const int* __restrict__ done) {
   if (done && *done) return;
-  __shared__ double sh[256][16];
+  __shared__ double sh[256][17];
   const long r = (long)blockIdx.x * 256 + threadIdx.x;
-  for (int w = 0; w < 16; w++) sh[threadIdx.x][w] = r < rows ? in[r * kPartialStride + w] : 0.0;
+  for (int w = 0; w < 17; w++) sh[threadIdx.x][w] = r < rows ? in[r * kPartialStride + w] : 0.0;
   __syncthreads();
   for (int stride = 128; stride > 0; stride >>= 1) {
     if (threadIdx.x < stride) {
-      for (int w = 0; w < 14; w++) sh[threadIdx.x][w] += sh[threadIdx.x + stride][w];
-      for (int w = 14; w < 16; w++) sh[threadIdx.x][w] = fmax(sh[threadIdx.x][w], sh[threadIdx.x + stride][w]);
+      for (int w = 0; w < 15; w++) sh[threadIdx.x][w] += sh[threadIdx.x + stride][w];
+      for (int w = 15; w < 17; w++) sh[threadIdx.x][w] = fmax(sh[threadIdx.x][w], sh[threadIdx.x + stride][w]);
     }
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int w = 0; w < 16; w++) out[(long)blockIdx.x * kPartialStride + w] = sh[0][w];
+    for (int w = 0; w < 17; w++) out[(long)blockIdx.x * kPartialStride + w] = sh[0][w];
 }
 
 // LoopState: the iteration counter and the done flag, both on the device so
@@ -355,19 +361,19 @@ __global__ void __launch_bounds__(256) k_iter_finalize(long n, int nblocks, cons
                                                        LoopState* ls, IterRecord* rec) {
   if (ls->done) return;
   const int it = ls->iter;
-  __shared__ double sh[256][16];
-  double acc[16];
-  for (int w = 0; w < 16; w++) acc[w] = 0.0;
+  __shared__ double sh[256][17];
+  double acc[17];
+  for (int w = 0; w < 17; w++) acc[w] = 0.0;
   for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
-    for (int w = 0; w < 14; w++) acc[w] += partial[(long)b * kPartialStride + w];
-    for (int w = 14; w < 16; w++) acc[w] = fmax(acc[w], partial[(long)b * kPartialStride + w]);
+    for (int w = 0; w < 15; w++) acc[w] += partial[(long)b * kPartialStride + w];
+    for (int w = 15; w < 17; w++) acc[w] = fmax(acc[w], partial[(long)b * kPartialStride + w]);
   }
-  for (int w = 0; w < 16; w++) sh[threadIdx.x][w] = acc[w];
+  for (int w = 0; w < 17; w++) sh[threadIdx.x][w] = acc[w];
   __syncthreads();
   for (int stride = 128; stride > 0; stride >>= 1) {
     if (threadIdx.x < stride) {
-      for (int w = 0; w < 14; w++) sh[threadIdx.x][w] += sh[threadIdx.x + stride][w];
-      for (int w = 14; w < 16; w++) sh[threadIdx.x][w] = fmax(sh[threadIdx.x][w], sh[threadIdx.x + stride][w]);
+      for (int w = 0; w < 15; w++) sh[threadIdx.x][w] += sh[threadIdx.x + stride][w];
+      for (int w = 15; w < 17; w++) sh[threadIdx.x][w] = fmax(sh[threadIdx.x][w], sh[threadIdx.x + stride][w]);
     }
     __syncthreads();
   }
@@ -390,8 +396,9 @@ __global__ void __launch_bounds__(256) k_iter_finalize(long n, int nblocks, cons
     R.err_strain_rate = ee;
     R.nr_steps = sh[0][12];
     R.degenerate = sh[0][13];
-    R.max_nr_error = sh[0][14];
-    R.max_nr_steps = sh[0][15];
+    R.nr_wave_steps = sh[0][14];
+    R.max_nr_error = sh[0][15];
+    R.max_nr_steps = sh[0][16];
     R.iteration = it + 1;
     // loop condition, MaterialGrid.cpp:1066-1068
     R.done = R.degenerate > 0 || !((it + 1) < max_iter && (es > tol || ee > tol));
