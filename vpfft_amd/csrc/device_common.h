@@ -236,101 +236,267 @@ __device__ __forceinline__ double sgn(double v) {  // Solvers.h:23-25
   return (double)((0.0 < v) - (v < 0.0));
 }
 
-// Solvers::SolveConstitutiveEquations (Taylor NR), Solvers.cpp:19-101.
-__device__ __forceinline__ void taylor_nr(const double init[5], const double m[kSlips][5],
-                                          const double inv_tau[kSlips], const StepConst& C,
-                                          double eps, int max_it, double out[5]) {
-  double cur[5], nw[5], saved[5];
-#pragma unroll
-  for (int i = 0; i < 5; i++) { cur[i] = init[i]; nw[i] = init[i]; saved[i] = 0.0; }
-  int rem = max_it;
-  const double max_rss = C.imposed_max_rss;
-  while (rem > 0) {
-    double rss[kSlips];
-    bool ok;
-    do {
-      ok = true;
-#pragma unroll
-      for (int k = 0; k < kSlips; k++) {
-        double r = dot5(m[k], cur) * inv_tau[k];
-        r = fabs(r) < 1e-10 ? 0.0 : r;
-        ok = ok && !(fabs(r) > max_rss);
-        rss[k] = r;
-      }
-      if (!ok) {
-#pragma unroll
-        for (int i = 0; i < 5; i++) cur[i] = saved[i] + (cur[i] - saved[i]) * 0.2;
-      }
-      rem--;
-      if (rem < 0) {
-#pragma unroll
-        for (int i = 0; i < 5; i++) out[i] = nw[i];
-        return;
-      }
-    } while (!ok);
-    double res[5] = {0, 0, 0, 0, 0};
-    double A[15];
-#pragma unroll
-    for (int e = 0; e < 15; e++) A[e] = 0.0;
-#pragma unroll
-    for (int k = 0; k < kSlips; k++) {
-      const double g = gamma_nm1_taylor(C, k, fabs(rss[k]));
-      const double gr = g * rss[k];
-      const double ck = (double)nexp_of(C, k) * g * inv_tau[k];
-#pragma unroll
-      for (int i = 0; i < 5; i++) {
-        res[i] += m[k][i] * gr;
-#pragma unroll
-        for (int j = 0; j <= i; j++) A[sidx(i, j)] += ck * m[k][i] * m[k][j];
-      }
-    }
-    // J = -A ; delta = J^-1 (res - edot) = A^-1 (edot - res)
-    double rhs[5], delta[5];
-#pragma unroll
-    for (int i = 0; i < 5; i++) rhs[i] = C.edot[i] - res[i];
-    ldlt_solve5(A, rhs, delta);
-    double sum[5];
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-      saved[i] = cur[i];
-      nw[i] = cur[i] + delta[i];
-      sum[i] = nw[i] + cur[i];
-    }
-    const double rel = 2.0 * norm5(delta) / norm5(sum);
-#pragma unroll
-    for (int i = 0; i < 5; i++) cur[i] = nw[i];
-    if (rel < eps) break;
-  }
-#pragma unroll
-  for (int i = 0; i < 5; i++) out[i] = nw[i];
+// ---------------------------------------------------------------------------
+// Crystal-frame Newton solve.
+//
+// r_k = m_k . sigma with m_k = proj5(R m0_k R^T) equals m0_k . (Q sigma),
+// Q_ab = <B_a, R^T B_b R> (any R: trace identities only), and
+// m_k = Q^T m0_k.  In that frame the twelve Schmid vectors are constants
+// with one zero each, m0_k,i = kSign[k][i] * w_i, w = (1/(2 sqrt 3), 1/2,
+// 1/(2 sqrt 3), ...), so the resolved shear stresses, the stress sum and the
+// Newton matrix are sums of signed terms.  The orientation is updated
+// additively and drifts off SO(3) (MaterialGrid.cpp:870-871), so Q is not
+// orthogonal: with P = Q^-1 the sample-frame system (L0^-1 + Q^T J_c Q) d = rhs
+// is (P^T L0^-1 P + J_c) d_c = P^T rhs with d = P d_c.  The iterate is kept in
+// both frames; the step norm and the back-track use the sample frame, as the
+// reference does.
+namespace cf {
+
+// FCC {111}<110> Schmid vectors of BurgerVectorBasis.h:16-72, 122-132 in the
+// deviatoric basis of LinearAlgebra.h:207-247, divided by w_i.
+__device__ __constant__ static const signed char kSign[kSlips][5] = {
+    {1, -1, 0, 1, 1},   {-1, -1, 1, 0, 1},  {-2, 0, 1, -1, 0}, {-1, 1, 0, -1, 1},
+    {-1, -1, -1, 0, -1}, {-2, 0, -1, -1, 0}, {-1, 1, 0, 1, 1},  {-1, -1, 1, 0, -1},
+    {-2, 0, 1, 1, 0},   {1, -1, 0, -1, 1},  {-1, -1, -1, 0, 1}, {-2, 0, -1, 1, 0}};
+__device__ __forceinline__ constexpr double w(int i) { return i == 1 ? 0.5 : 0.28867513459481287; }
+
+// proj5 of a symmetric 3x3 given by entries (00,11,22,12,02,01)
+__device__ __forceinline__ void proj_sym(double c00, double c11, double c22, double c12, double c02, double c01,
+                                         double v[5]) {
+  const double s2 = 0.70710678118654752440, s6 = 0.40824829046386301637;
+  v[0] = (c11 - c00) * s2;
+  v[1] = (2.0 * c22 - c00 - c11) * s6;
+  v[2] = 2.0 * c12 * s2;
+  v[3] = 2.0 * c02 * s2;
+  v[4] = 2.0 * c01 * s2;
 }
 
-// Solvers::SolveConstrainedConstitutiveEquations, Solvers.cpp:111-222.
-// q = L0^-1 lambda + edot + dtilde (constant over the solve).  *err_out is
-// left untouched when the budget runs out inside the back-track (:166-170).
-// Register-lean form: resolved shear stresses are recomputed where needed
-// rather than kept (12 doubles), the iterate doubles as the returned value,
-// and the 5x5 system is factored in place.
-__device__ __forceinline__ void constrained_nr(double cur[5], const double m[kSlips][5],
-                                               const double inv_tau[kSlips], const StepConst& C,
-                                               const double q[5], double eps, double max_rss,
-                                               int max_it, double* err_out, int* steps_out) {
-  double saved[5];
-  double rel = 1e5;
+// Q(M)_ab = <B_a, M^T B_b M> (row-major, Q[5a+b]) for a 3x3 M (row-major):
+// column b is proj5(M^T B_b M); with rho_i the rows of M,
+// M^T e_i e_j^T M = rho_i rho_j^T.
+__device__ __forceinline__ void dev_map(const double M[9], double Q[25]) {
+  const double s2 = 0.70710678118654752440, s6 = 0.40824829046386301637;
+  const double* r0 = M;
+  const double* r1 = M + 3;
+  const double* r2 = M + 6;
+  auto sq = [](const double* a, int k, int l) { return a[k] * a[l]; };
+  auto mix = [](const double* a, const double* b, int k, int l) { return a[k] * b[l] + b[k] * a[l]; };
+  double col[5];
+  auto put = [&](int bcol) {
+#pragma unroll
+    for (int a = 0; a < 5; a++) Q[a * 5 + bcol] = col[a];
+  };
+  proj_sym((sq(r1, 0, 0) - sq(r0, 0, 0)) * s2, (sq(r1, 1, 1) - sq(r0, 1, 1)) * s2,
+           (sq(r1, 2, 2) - sq(r0, 2, 2)) * s2, (sq(r1, 1, 2) - sq(r0, 1, 2)) * s2,
+           (sq(r1, 0, 2) - sq(r0, 0, 2)) * s2, (sq(r1, 0, 1) - sq(r0, 0, 1)) * s2, col);
+  put(0);
+  proj_sym((2.0 * sq(r2, 0, 0) - sq(r0, 0, 0) - sq(r1, 0, 0)) * s6,
+           (2.0 * sq(r2, 1, 1) - sq(r0, 1, 1) - sq(r1, 1, 1)) * s6,
+           (2.0 * sq(r2, 2, 2) - sq(r0, 2, 2) - sq(r1, 2, 2)) * s6,
+           (2.0 * sq(r2, 1, 2) - sq(r0, 1, 2) - sq(r1, 1, 2)) * s6,
+           (2.0 * sq(r2, 0, 2) - sq(r0, 0, 2) - sq(r1, 0, 2)) * s6,
+           (2.0 * sq(r2, 0, 1) - sq(r0, 0, 1) - sq(r1, 0, 1)) * s6, col);
+  put(1);
+  proj_sym(mix(r1, r2, 0, 0) * s2, mix(r1, r2, 1, 1) * s2, mix(r1, r2, 2, 2) * s2, mix(r1, r2, 1, 2) * s2,
+           mix(r1, r2, 0, 2) * s2, mix(r1, r2, 0, 1) * s2, col);
+  put(2);
+  proj_sym(mix(r0, r2, 0, 0) * s2, mix(r0, r2, 1, 1) * s2, mix(r0, r2, 2, 2) * s2, mix(r0, r2, 1, 2) * s2,
+           mix(r0, r2, 0, 2) * s2, mix(r0, r2, 0, 1) * s2, col);
+  put(3);
+  proj_sym(mix(r0, r1, 0, 0) * s2, mix(r0, r1, 1, 1) * s2, mix(r0, r1, 2, 2) * s2, mix(r0, r1, 1, 2) * s2,
+           mix(r0, r1, 0, 2) * s2, mix(r0, r1, 0, 1) * s2, col);
+  put(4);
+}
+
+// P = Q(R)^-1.  Q(R) maps X to dev(R^T X R) on deviatoric symmetric tensors;
+// its inverse maps Y to R^-T Y R^-1 + alpha H, H = R^-T R^-1, with alpha
+// making the result traceless: P = Q(R^-1) - h k^T / tr H, h = proj5(H),
+// k = proj5(R^-1 R^-T).  For a rotation P = Q^T.
+__device__ __forceinline__ void inverse_map(const double R[9], double P[25]) {
+  // R^-1 = adj(R) / det(R)
+  const double c00 = R[4] * R[8] - R[5] * R[7], c01 = R[5] * R[6] - R[3] * R[8], c02 = R[3] * R[7] - R[4] * R[6];
+  const double det = R[0] * c00 + R[1] * c01 + R[2] * c02;
+  const double id = 1.0 / det;
+  double Ri[9];
+  Ri[0] = c00 * id;
+  Ri[3] = c01 * id;
+  Ri[6] = c02 * id;
+  Ri[1] = (R[2] * R[7] - R[1] * R[8]) * id;
+  Ri[4] = (R[0] * R[8] - R[2] * R[6]) * id;
+  Ri[7] = (R[1] * R[6] - R[0] * R[7]) * id;
+  Ri[2] = (R[1] * R[5] - R[2] * R[4]) * id;
+  Ri[5] = (R[2] * R[3] - R[0] * R[5]) * id;
+  Ri[8] = (R[0] * R[4] - R[1] * R[3]) * id;
+  dev_map(Ri, P);
+  // H = Ri^T Ri (columns of Ri), K = Ri Ri^T (rows of Ri)
+  auto hcol = [&](int i, int j) { return Ri[i] * Ri[j] + Ri[3 + i] * Ri[3 + j] + Ri[6 + i] * Ri[6 + j]; };
+  auto krow = [&](int i, int j) { return Ri[3 * i] * Ri[3 * j] + Ri[3 * i + 1] * Ri[3 * j + 1] + Ri[3 * i + 2] * Ri[3 * j + 2]; };
+  double h[5], k[5];
+  const double h00 = hcol(0, 0), h11 = hcol(1, 1), h22 = hcol(2, 2);
+  proj_sym(h00, h11, h22, hcol(1, 2), hcol(0, 2), hcol(0, 1), h);
+  proj_sym(krow(0, 0), krow(1, 1), krow(2, 2), krow(1, 2), krow(0, 2), krow(0, 1), k);
+  const double itr = 1.0 / (h00 + h11 + h22);
+#pragma unroll
+  for (int a = 0; a < 5; a++) {
+    const double ha = h[a] * itr;
+#pragma unroll
+    for (int bcol = 0; bcol < 5; bcol++) P[a * 5 + bcol] -= ha * k[bcol];
+  }
+}
+
+// y = Q x  and  y = Q^T x
+__device__ __forceinline__ void rot(const double Q[25], const double x[5], double y[5]) {
+#pragma unroll
+  for (int a = 0; a < 5; a++) {
+    double t = 0.0;
+#pragma unroll
+    for (int b = 0; b < 5; b++) t += Q[a * 5 + b] * x[b];
+    y[a] = t;
+  }
+}
+__device__ __forceinline__ void rot_t(const double Q[25], const double x[5], double y[5]) {
+#pragma unroll
+  for (int b = 0; b < 5; b++) {
+    double t = 0.0;
+#pragma unroll
+    for (int a = 0; a < 5; a++) t += Q[a * 5 + b] * x[a];
+    y[b] = t;
+  }
+}
+
+// P^T A P for a symmetric A given column-major (the constant L0^-1); packed lower result.
+__device__ __forceinline__ void congruence_t(const double P[25], const double* __restrict__ Acm, double out[15]) {
+  double T[25];  // T = A P
+#pragma unroll
+  for (int i = 0; i < 5; i++)
+#pragma unroll
+    for (int c = 0; c < 5; c++) {
+      double t = 0.0;
+#pragma unroll
+      for (int j = 0; j < 5; j++) t += Acm[j * 5 + i] * P[j * 5 + c];
+      T[i * 5 + c] = t;
+    }
+#pragma unroll
+  for (int a = 0; a < 5; a++)
+#pragma unroll
+    for (int c = 0; c <= a; c++) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < 5; i++) t += P[i * 5 + a] * T[i * 5 + c];
+      out[sidx(a, c)] = t;
+    }
+}
+
+// m0_k . x, exact sums of signed (and doubled) terms of u = w * x
+__device__ __forceinline__ double schmid_dot(int k, const double u[5]) {
+  double d = -0.0;  // -0.0 + t == t, so the first term costs nothing
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    const int sg = kSign[k][i];
+    if (sg == 1) d += u[i];
+    if (sg == -1) d -= u[i];
+    if (sg == 2) d += 2.0 * u[i];
+    if (sg == -2) d -= 2.0 * u[i];
+  }
+  return d;
+}
+
+__device__ __forceinline__ void scaled(const double x[5], double u[5]) {
+#pragma unroll
+  for (int i = 0; i < 5; i++) u[i] = w(i) * x[i];
+}
+
+// x^E by a fixed squaring chain (E known at compile time)
+template <int E>
+__device__ __forceinline__ double ipow_c(double x) {
+  if constexpr (E == 0) {
+    return 1.0;
+  } else if constexpr (E == 1) {
+    return x;
+  } else {
+    const double h = ipow_c<E / 2>(x);
+    return (E & 1) ? h * h * x : h * h;
+  }
+}
+
+// gdot0 |r|^(n-1) (Solvers.cpp:176-180); NE > 0: every slip has exponent NE.
+template <int NE>
+__device__ __forceinline__ double gamma_nm1(const StepConst& C, int k, double absr) {
+  if constexpr (NE > 0) {
+    return NE != 1 ? C.gdot0[k] * ipow_c<NE - 1>(absr) : C.gdot0[k];
+  } else {
+    return vpfft::gamma_nm1(C, k, absr);
+  }
+}
+template <int NE>
+__device__ __forceinline__ double gamma_n(const StepConst& C, int k, double absr) {
+  if constexpr (NE > 0) {
+    return C.gdot0[k] * ipow_c<NE>(absr);
+  } else {
+    return vpfft::gamma_n(C, k, absr);
+  }
+}
+template <int NE>
+__device__ __forceinline__ double nexp(const StepConst& C, int k) {
+  if constexpr (NE > 0) {
+    return (double)NE;
+  } else {
+    return (double)nexp_of(C, k);
+  }
+}
+
+// max_k |m0_k . sc| / tau_k (MaterialGrid.h:550-566)
+__device__ __forceinline__ double max_rss(const double sc[5], const double inv_tau[kSlips]) {
+  double u[5];
+  scaled(sc, u);
+  double mx = 0.0;
+#pragma unroll
+  for (int k = 0; k < kSlips; k++) mx = fmax(mx, fabs(schmid_dot(k, u) * inv_tau[k]));
+  return mx;
+}
+
+// Solvers::SolveConstrainedConstitutiveEquations (Solvers.cpp:111-222) with
+// the Newton matrix built in the crystal frame.  cur: the sample-frame
+// iterate (in/out); R: the orientation (Q = Q(R) is rebuilt on a back-track);
+// P = Q^-1; q = P^T (L0^-1 lambda + edot + d~); A0 = P^T L0^-1 P (packed
+// lower).  Budget, back-track and stopping rule as in the reference;
+// *err_out untouched when the budget runs out in the back-track.
+template <int NE>
+__device__ __forceinline__ void constrained_nr(double cur[5], const double R[9], const double P[25],
+                                               const double inv_tau[kSlips], const StepConst& C, const double q[5],
+                                               const double A0[15], double eps, double max_rss, int max_it,
+                                               double* err_out, int* steps_out) {
+  double cc[5], saved[5];
+  {
+    double Q[25];
+    dev_map(R, Q);
+    rot(Q, cur, cc);
+  }
 #pragma unroll
   for (int i = 0; i < 5; i++) saved[i] = cur[i];
   int rem = max_it;
   int steps = 0;
+  double dn = 1.0, an = 1e-10;  // rel = 1e5 until a step is taken (Solvers.cpp:137)
+  const double eps2 = eps * eps;
   while (rem > 0) {
+    double r[kSlips];
     bool ok;
     do {
+      double u[5];
+      scaled(cc, u);
       double mx = 0.0;
 #pragma unroll
-      for (int k = 0; k < kSlips; k++) mx = fmax(mx, fabs(dot5(m[k], cur) * inv_tau[k]));
+      for (int k = 0; k < kSlips; k++) {
+        r[k] = schmid_dot(k, u) * inv_tau[k];
+        mx = fmax(mx, fabs(r[k]));
+      }
       ok = !(mx > max_rss);
       if (!ok) {
 #pragma unroll
         for (int i = 0; i < 5; i++) cur[i] = saved[i] + (cur[i] - saved[i]) * 0.2;
+        double Q[25];
+        dev_map(R, Q);
+        rot(Q, cur, cc);
       }
       rem--;
       if (rem < 0) {
@@ -339,77 +505,98 @@ __device__ __forceinline__ void constrained_nr(double cur[5], const double m[kSl
       }
     } while (!ok);
     steps++;
-    double S[5] = {0, 0, 0, 0, 0};
-    double A[15];
+    // S_c = sum_k m0_k gdot_k r_k ; J_c = sum_k n gdot_k / tau_k m0_k m0_k^T
+    double S[5], J[15];
 #pragma unroll
-    for (int i = 0, e = 0; i < 5; i++)
+    for (int i = 0; i < 5; i++) S[i] = -0.0;
 #pragma unroll
-      for (int j = 0; j <= i; j++, e++) A[e] = C.L0inv[j * 5 + i];
+    for (int e = 0; e < 15; e++) J[e] = -0.0;
 #pragma unroll
     for (int k = 0; k < kSlips; k++) {
-      const double r = dot5(m[k], cur) * inv_tau[k];
-      const double g = gamma_nm1(C, k, fabs(r));
-      const double gr = g * r;
-      const double ck = (double)nexp_of(C, k) * g * inv_tau[k];
+      const double g = gamma_nm1<NE>(C, k, fabs(r[k]));
+      const double gr = g * r[k];
+      const double ck = nexp<NE>(C, k) * g * inv_tau[k];
 #pragma unroll
       for (int i = 0; i < 5; i++) {
-        S[i] += m[k][i] * gr;
-        const double cm = ck * m[k][i];
+        const int si = kSign[k][i];
+        if (si == 0) continue;
+        S[i] += (double)si * gr;
 #pragma unroll
-        for (int j = 0; j <= i; j++) A[sidx(i, j)] += cm * m[k][j];
+        for (int j = 0; j <= i; j++) {
+          const int sij = si * kSign[k][j];  // 0, +-1, +-2 or 4
+          if (sij == 0) continue;
+          if (sij == 1) {
+            J[sidx(i, j)] += ck;
+          } else if (sij == -1) {
+            J[sidx(i, j)] -= ck;
+          } else {
+            J[sidx(i, j)] += (double)sij * ck;
+          }
+        }
       }
     }
-    // L0^-1 Res = L0^-1 cur + S - q  ;  delta = -(L0^-1 + J)^-1 L0^-1 Res
+    // rhs_c = q - A0 cc - S_c ;  (A0 + J_c) d_c = rhs_c ; d = P d_c
     double rhs[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) {
-      double Lc = 0.0;
+      double Ac = 0.0;
 #pragma unroll
-      for (int j = 0; j < 5; j++) Lc += C.L0inv[j * 5 + i] * cur[j];
-      rhs[i] = q[i] - Lc - S[i];
+      for (int j = 0; j < 5; j++) Ac += A0[i >= j ? sidx(i, j) : sidx(j, i)] * cc[j];
+      rhs[i] = q[i] - Ac - w(i) * S[i];
     }
-    ldlt_solve5_inplace(A, rhs);
-    double dn = 0.0, an = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+#pragma unroll
+      for (int j = 0; j <= i; j++) J[sidx(i, j)] = A0[sidx(i, j)] + (w(i) * w(j)) * J[sidx(i, j)];
+    ldlt_solve5_inplace(J, rhs);
+    double ds[5];
+    rot(P, rhs, ds);
+    dn = 0.0;
+    an = 0.0;
 #pragma unroll
     for (int i = 0; i < 5; i++) {
       saved[i] = cur[i];
-      const double nw = cur[i] + rhs[i];
+      const double nw = cur[i] + ds[i];
       const double diff = nw - saved[i];
       const double avg = (nw + saved[i]) * 0.5;
       dn += diff * diff;
       an += avg * avg;
       cur[i] = nw;
+      cc[i] += rhs[i];
     }
-    rel = sqrt(dn) / sqrt(an);
-    if (rel < eps) break;
+    // rel = |d| / |avg| < eps, compared squared; rel itself is formed once at exit
+    if (dn < eps2 * an) break;
   }
-  *err_out = rel;
+  *err_out = sqrt(dn) / sqrt(an);
   *steps_out = steps;
 }
 
-// Solvers::ApplyConstitutiveEquations, Solvers.cpp:255-274.
-__device__ __forceinline__ void apply_constitutive(const double s[5], const double m[kSlips][5],
-                                                   const double inv_tau[kSlips],
-                                                   const StepConst& C, double e[5]) {
+// Solvers::ApplyConstitutiveEquations (Solvers.cpp:255-274), crystal frame.
+template <int NE>
+__device__ __forceinline__ void apply_constitutive(const double sc[5], const double inv_tau[kSlips],
+                                                   const StepConst& C, double ec[5]) {
+  double u[5], a[5];
+  scaled(sc, u);
 #pragma unroll
-  for (int i = 0; i < 5; i++) e[i] = 0.0;
+  for (int i = 0; i < 5; i++) a[i] = -0.0;
 #pragma unroll
   for (int k = 0; k < kSlips; k++) {
-    const double r = dot5(m[k], s) * inv_tau[k];
-    const double g = gamma_n(C, k, fabs(r)) * sgn(r);
+    const double r = schmid_dot(k, u) * inv_tau[k];
+    const double g = gamma_n<NE>(C, k, fabs(r)) * sgn(r);
 #pragma unroll
-    for (int i = 0; i < 5; i++) e[i] += m[k][i] * g;
+    for (int i = 0; i < 5; i++) {
+      const int si = kSign[k][i];
+      if (si == 1) a[i] += g;
+      if (si == -1) a[i] -= g;
+      if (si == 2) a[i] += 2.0 * g;
+      if (si == -2) a[i] -= 2.0 * g;
+    }
   }
+#pragma unroll
+  for (int i = 0; i < 5; i++) ec[i] = w(i) * a[i];
 }
 
-// max_k |m_k . s / tau_k|  (MaterialGrid.h:550-566)
-__device__ __forceinline__ double max_rss(const double s[5], const double m[kSlips][5],
-                                          const double inv_tau[kSlips]) {
-  double mx = 0.0;
-#pragma unroll
-  for (int k = 0; k < kSlips; k++) mx = fmax(mx, fabs(dot5(m[k], s) * inv_tau[k]));
-  return mx;
-}
+}  // namespace cf
 
 // ---------------------------------------------------------------------------
 // Bit-faithful prologue path.

@@ -10,6 +10,7 @@ namespace vpfft {
 // Fixed launch shape of every pointwise kernel: the voxel->thread assignment
 // and therefore the order of every reduction is independent of timing.
 constexpr int kPointThreads = 256;
+constexpr int kNrWaves = 2;        // waves per SIMD the Newton kernel is compiled for
 constexpr int kPartialStride = 24;  // 15 sums + 2 maxima per block, padded
 
 // the Newton kernel runs one voxel per lane
@@ -58,7 +59,7 @@ hipError_t launch_L0_finalize(long n, int nblocks, const double* partial, StepCo
 hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32, hipStream_t st);
 // waves: 1 or 2 waves per SIMD (register budget 512 or 256 per lane);
 // uniform_tau: every slip system of a voxel has the same CRSS (crss[0][v] is read)
-hipError_t launch_nr(int prec, int waves, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
+hipError_t launch_nr(int prec, int nexp_uniform, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
                      double* lambda, const double* R, const double* crss, const StepConst* C, double* partial,
                      const int* done, hipStream_t st);
 hipError_t launch_iter_finalize(long n, const double* partial, const StepConst* C, double tol,

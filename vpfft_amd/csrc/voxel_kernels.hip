@@ -239,18 +239,18 @@ __global__ void k_derive(StepConst* C, GreenConst<double>* g64, GreenConst<float
 }
 
 // ---------------------------------------------------------------------------
-// K4: per-voxel body of the fixed-point loop.
+// K4: per-voxel body of the fixed-point loop (MaterialGrid.cpp:946-1003),
+// with the Newton solve in the crystal frame of the voxel.
 // partial per block: [0..4] sum sigma, [5..9] sum edot, [10] sum|sigma-lambda|,
-// [11] sum|edot-E-d|, [12] NR steps, [13] degenerate, [14] max NR error
+// [11] sum|edot-E-d|, [12] NR steps, [13] degenerate, [14] sum of 64 x
+// per-wave max NR steps; then max NR error, max NR steps
+// NE > 0: every slip system has rate-sensitivity exponent NE.
 // ---------------------------------------------------------------------------
-template <typename T, int WAVES, bool UNIFORM_TAU>
-__global__ void __launch_bounds__(kPointThreads) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
-k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
-                                                      double* __restrict__ sigma,
-                                                      double* __restrict__ lambda, const double* __restrict__ R,
-                                                      const double* __restrict__ crss,
-                                                      const StepConst* __restrict__ Cp,
-                                                      double* __restrict__ partial, const int* __restrict__ done) {
+template <typename T, int NE, bool UNIFORM_TAU>
+__global__ void __launch_bounds__(kPointThreads) __attribute__((amdgpu_waves_per_eu(kNrWaves, kNrWaves)))
+k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi, double* __restrict__ sigma,
+     double* __restrict__ lambda, const double* __restrict__ R, const double* __restrict__ crss,
+     const StepConst* __restrict__ Cp, double* __restrict__ partial, const int* __restrict__ done) {
   if (done && *done) return;
   const StepConst& C = *Cp;
   double acc[15];
@@ -260,12 +260,8 @@ k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
   // one voxel per lane: nothing but the voxel's own state is live
   const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (v < n) {
-    double it[kSlips], m[kSlips][5], s[5], q[5];
-    {
-      double r[9];
-      load_R(R, n, v, r);
-      schmid_from_R(r, m);
-    }
+    double it[kSlips], r9[9];
+    load_R(R, n, v, r9);
     if (UNIFORM_TAU) {
       // every slip system of the voxel has the same CRSS (one load)
       const double t = 1.0 / crss[v];
@@ -274,13 +270,22 @@ k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
     } else {
       load_inv_tau(crss, n, v, it);
     }
+    double s[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) s[i] = sigma[i * n + v];
-    const double mrss = max_rss(s, m, it);
+    // yield-surface rescale, MaterialGrid.cpp:954-959
+    double mrss;
+    {
+      double Q[25], sc[5];
+      cf::dev_map(r9, Q);
+      cf::rot(Q, s, sc);
+      mrss = cf::max_rss(sc, it);
+    }
     if (!(mrss >= 1e-10)) acc[13] += 1.0;
 #pragma unroll
     for (int i = 0; i < 5; i++) s[i] /= mrss;
     // q = L0^-1 lambda + edot + d~ ; lambda and d~ are re-read after the solve
+    double q[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) q[i] = C.edot[i] + ld(dtil + i * n + v);
 #pragma unroll
@@ -291,9 +296,21 @@ k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi,
     }
     double err = 0.0;
     int steps = 0;
-    constrained_nr(s, m, it, C, q, 5e-4, 1e9, 100, &err, &steps);
+    {
+      double P[25], qc[5], A0[15];
+      cf::inverse_map(r9, P);
+      cf::rot_t(P, q, qc);
+      cf::congruence_t(P, C.L0inv, A0);
+      cf::constrained_nr<NE>(s, r9, P, it, C, qc, A0, 5e-4, 1e9, 100, &err, &steps);
+    }
     double e[5];
-    apply_constitutive(s, m, it, C, e);
+    {
+      double Q[25], sc[5], ec[5];
+      cf::dev_map(r9, Q);
+      cf::rot(Q, s, sc);
+      cf::apply_constitutive<NE>(sc, it, C, ec);
+      cf::rot_t(Q, ec, e);
+    }
     double d[5], lam[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) {
@@ -510,17 +527,19 @@ __global__ void k_constrained_batch(long n, const double* __restrict__ R, const 
                                     double* __restrict__ s_out, double* __restrict__ err, int* __restrict__ steps) {
   const StepConst& C = *Cp;
   for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
-    double r[9], it[kSlips], m[kSlips][5], s[5], l[5], d[5];
+    double r[9], it[kSlips], P[25], s[5], l[5], d[5];
     for (int e = 0; e < 9; e++) r[e] = R[9 * v + e];
     for (int k = 0; k < kSlips; k++) it[k] = 1.0 / crss[12 * v + k];
     for (int i = 0; i < 5; i++) { s[i] = s_in[5 * v + i]; l[i] = lam[5 * v + i]; d[i] = dtl[5 * v + i]; }
-    schmid_from_R(r, m);
-    double Ll[5], q[5];
+    cf::inverse_map(r, P);
+    double Ll[5], q[5], qc[5], A0[15];
     matvec5(C.L0inv, l, Ll);
     for (int i = 0; i < 5; i++) q[i] = Ll[i] + C.edot[i] + d[i];
+    cf::rot_t(P, q, qc);
+    cf::congruence_t(P, C.L0inv, A0);
     double e = err[v];
     int st = 0;
-    constrained_nr(s, m, it, C, q, 5e-4, 1e9, 100, &e, &st);
+    cf::constrained_nr<0>(s, r, P, it, C, qc, A0, 5e-4, 1e9, 100, &e, &st);
     for (int i = 0; i < 5; i++) s_out[5 * v + i] = s[i];
     err[v] = e;
     steps[v] = st;
@@ -566,22 +585,26 @@ hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenCon
   return hipGetLastError();
 }
 
-hipError_t launch_nr(int prec, int waves, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
+hipError_t launch_nr(int prec, int nexp_uniform, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
                      double* lambda, const double* R, const double* crss, const StepConst* C, double* partial,
                      const int* done, hipStream_t st) {
   const int nb = nr_blocks(n);
-#define NR_LAUNCH(TT, WW, UU)                                                                                    \
-  hipLaunchKernelGGL((k_nr<TT, WW, UU>), dim3(nb), dim3(kPointThreads), 0, st, n, (const TT*)dtil, (TT*)phi, sigma, \
+#define NR_LAUNCH(TT, NN, UU)                                                                                    \
+  hipLaunchKernelGGL((k_nr<TT, NN, UU>), dim3(nb), dim3(kPointThreads), 0, st, n, (const TT*)dtil, (TT*)phi, sigma, \
                      lambda, R, crss, C, partial, done)
-#define NR_W(TT, UU)                   \
-  if (waves >= 2) NR_LAUNCH(TT, 2, UU); \
-  else NR_LAUNCH(TT, 1, UU);
-  if (prec == 0) {
-    if (uniform_tau) { NR_W(double, true) } else { NR_W(double, false) }
-  } else {
-    if (uniform_tau) { NR_W(float, true) } else { NR_W(float, false) }
+#define NR_N(TT, UU)                                 \
+  switch (nexp_uniform) {                             \
+    case 5: NR_LAUNCH(TT, 5, UU); break;              \
+    case 10: NR_LAUNCH(TT, 10, UU); break;            \
+    case 20: NR_LAUNCH(TT, 20, UU); break;            \
+    default: NR_LAUNCH(TT, 0, UU); break;             \
   }
-#undef NR_W
+  if (prec == 0) {
+    if (uniform_tau) { NR_N(double, true) } else { NR_N(double, false) }
+  } else {
+    if (uniform_tau) { NR_N(float, true) } else { NR_N(float, false) }
+  }
+#undef NR_N
 #undef NR_LAUNCH
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
