@@ -134,6 +134,89 @@ def cpu_baseline(V, args, edot):
     }
 
 
+def batched_workload(V, args):
+    """Uneven RVEs advancing together: edges drawn from {16,24,32,48,64,96},
+    grains log-uniform in 8-512, n from {5,10,20}, each its own random unit
+    traceless strain rate; FP32 FFT + FP64 Newton; tuned FFT block shapes;
+    several strain steps of dt 0.005.  Timed once as one batch and once
+    grid by grid through the single-grid entry point."""
+    rng = np.random.default_rng(args.seed + 4)
+    cases = []
+    for i in range(args.batch_rves):
+        e = int(rng.choice([16, 24, 32, 48, 64, 96]))
+        grains = int(round(float(np.exp(rng.uniform(np.log(8.0), np.log(512.0))))))
+        nexp = int(rng.choice([5, 10, 20]))
+        ed = rng.standard_normal(5)
+        cases.append((e, grains, nexp, ed / np.linalg.norm(ed), 1000 + i))
+    micro = [V.generate_voronoi(e, e, e, gr, sd)[0] for (e, gr, _, _, sd) in cases]
+
+    def make():
+        gs = []
+        for (e, gr, nexp, _, _), R in zip(cases, micro):
+            g = V.MaterialGrid(e, e, e, V.Material.uniform(1.0, nexp), precision=V.MIXED)
+            g.SetVoxelLength(1.0, 1.0, 1.0)
+            g.SendDataAndInitialize(R)
+            gs.append(g)
+        return gs
+
+    edots = np.array([c[3] for c in cases])
+    voxels = sum(c[0] ** 3 for c in cases)
+    with V.Batch(tune=True) as b:
+        gs = make()  # warm-up pass: tunes the block shapes, loads the kernels
+        b.RunSingleStrainStep(gs, edots, args.max_iter, args.dt, args.batch_tol)
+        for g in gs:
+            g.close()
+        gs = make()
+        iters = 0
+        t0 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            b.RunSingleStrainStep(gs, edots, args.max_iter, args.dt, args.batch_tol)
+            iters += sum(inf.iterations * c[0] ** 3 for inf, c in zip(b.last_infos, cases))
+        wall = time.perf_counter() - t0
+        shapes = sorted({(c[0], tuple(b.block_shape(g, p) for p in range(3))) for g, c in zip(gs, cases)})
+        for g in gs:
+            g.close()
+    gs = make()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        for g, ed in zip(gs, edots):
+            g.RunSingleStrainStep(ed, args.max_iter, args.dt, args.batch_tol)
+    wall_seq = time.perf_counter() - t0
+    for g in gs:
+        g.close()
+    rve_steps = args.batch_rves * args.batch_steps
+    return {
+        "rves": args.batch_rves, "voxels": voxels, "steps": args.batch_steps, "tol": args.batch_tol,
+        "precision": "fp32 FFT + fp64 Newton",
+        "rve_steps_per_s": rve_steps / wall, "voxel_iters_per_s": iters / wall, "wall_s": wall,
+        "one_by_one_rve_steps_per_s": rve_steps / wall_seq, "speedup_vs_one_by_one": wall_seq / wall,
+        "tuned_block_shapes_zyx": {str(e): list(sh) for e, sh in shapes},
+    }
+
+
+def serving_workload(V, args):
+    """Poisson request streams at a few rates: each request one RVE (edge
+    32-128 in steps of 16, 8-256 grains, n in {10, 20}) and 1-10 increments;
+    admitted by an HBM budget, resident between increments, mixed precision
+    with FP64 reruns of steps that do not converge."""
+    from vpfft_amd import serve as S
+    out = []
+    for rate in args.serve_rates:
+        reqs = S.poisson_requests(args.serve_requests, rate, seed=args.seed + 11)
+        res, st, _ = S.serve(reqs, hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)), max_active=64,
+                             precision=V.MIXED, fp64_fallback=True, max_iter=args.max_iter, tol=args.batch_tol)
+        bad = sum(1 for r in res if r["status"] != 0)
+        out.append({"rate_per_s": rate, "requests": len(reqs), "failed": bad,
+                    "rve_steps_per_s": st["rve_steps_per_s"], "voxel_iters_per_s": st["voxel_iters_per_s"],
+                    "p50_latency_s": st["p50_latency_s"], "p99_latency_s": st["p99_latency_s"],
+                    "mean_latency_s": st["mean_latency_s"], "wall_s": st["wall_s"],
+                    "max_active": st["max_active_seen"], "max_gib": st["max_bytes_seen"] / (1 << 30),
+                    "batched_steps": st["batched_steps"], "fp64_reruns": st["fp64_reruns"]})
+        log(f"[bench] serving at {rate}/s: p50 {st['p50_latency_s']:.3f} s, p99 {st['p99_latency_s']:.3f} s, "
+            f"{st['rve_steps_per_s']:.1f} RVE-steps/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +234,14 @@ def main():
     ap.add_argument("--cpu-size", type=int, default=96)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-max-iters", type=int, default=20)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-rves", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-tol", type=float, default=1e-5)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[4.0, 16.0])
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only copy this many MiB device to device (torch) three times and exit: a kernel of "
                          "known traffic that calibrates FETCH_SIZE / WRITE_SIZE counters read under rocprofv3")
@@ -227,6 +318,18 @@ def main():
                  "macro_stress_rel_err_step1": err, "ms_per_step": 1e3 * mx["wall"] / args.steps}
         log(f"[bench] mixed: {mx['iters']} iterations in {mx['wall']:.3f} s, <sigma> err {err:.2e}")
 
+    batched = None
+    if not args.no_batched:
+        log(f"[bench] batched: {args.batch_rves} uneven RVEs")
+        batched = batched_workload(V, args)
+        log(f"[bench] batched: {batched['rve_steps_per_s']:.1f} RVE-steps/s, "
+            f"{batched['speedup_vs_one_by_one']:.2f}x the one-by-one rate")
+
+    serving = None
+    if not args.no_serving:
+        log("[bench] serving: Poisson request streams")
+        serving = serving_workload(V, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle)")
@@ -263,6 +366,8 @@ def main():
         "roofline": roof,
         "kernels": per_kernel,
         "mixed": mixed,
+        "batched": batched,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

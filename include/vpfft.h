@@ -173,6 +173,45 @@ int vpfft_solve_taylor_batch(long n, const vpfft_material* mat, const double* or
 int vpfft_generate_voronoi(int nx, int ny, int nz, int num_grains, uint64_t seed, double* orientation_out,
                            int32_t* grain_out, double* euler_out);
 
+/* Device bytes vpfft_grid_create allocates for desc (no reference
+ * counterpart): what an admission policy charges against an HBM budget. */
+int vpfft_grid_bytes(const vpfft_grid_desc* desc, uint64_t* bytes_out);
+/* Device-side copy of the state a strain step changes -- orientation, CRSS,
+ * accumulated shear, stress, Lagrange multiplier, voxel lengths -- and its
+ * restoration (no reference counterpart): lets a caller rerun a step, e.g. in
+ * FP64 after a mixed-precision step that did not converge.  The copy is
+ * allocated on first use (32 doubles per voxel). */
+int vpfft_grid_save_state(void* grid);
+int vpfft_grid_restore_state(void* grid);
+
+/* ---- batched strain steps (no reference counterpart: the reference runs one
+ * grid per MPI job).  A batch advances several grids -- independent RVEs of
+ * any sizes, materials and precisions -- through one RunSingleStrainStep each,
+ * together: every pass of the fixed-point loop is one launch per group of
+ * grids sharing a kernel variant, and each grid keeps its own reference
+ * medium, convergence test and iteration count.  A grid's result is the same
+ * bits whether it runs alone (vpfft_run_strain_step excepted: its reductions
+ * are grouped differently) or in any batch.  Grids must be uploaded; a grid
+ * may be in one batch call at a time; calls on one batch are serialised by
+ * the caller. */
+int vpfft_batch_create(void** out_batch);
+int vpfft_batch_destroy(void* batch);
+/* enable = 1: the FFT block shape of each (precision, pass, grid shape) is
+ * timed over the candidates once and the fastest kept (the arithmetic does
+ * not depend on it); 0: fixed defaults. */
+int vpfft_batch_set_tuning(void* batch, int enable);
+/* The block shape (pencils per block) a batch uses for grid's pass
+ * (0 = z, 1 = y, 2 = x). */
+int vpfft_batch_block_shape(void* batch, const void* grid, int pass, int* block_out);
+/* One strain step of count grids.  edot: count x 5; dt: count time steps
+ * (dt <= 0: no orientation / hardening update for that grid, as in
+ * vpfft_run_strain_step); avg_stress_out: count x 5 (may be NULL); infos:
+ * count (may be NULL).  Per-grid errors (degenerate voxels, non-finite
+ * results) are reported in infos[j].status; the return value is the first
+ * of them. */
+int vpfft_batch_run_strain_step(void* batch, void* const* grids, int count, const double* edot, int max_iter,
+                                const double* dt, double tol, double* avg_stress_out, vpfft_step_info* infos);
+
 /* Per-kernel device time (no reference counterpart; the reference has no
  * timers, README.make:40-43).  When enabled, every kernel launch of a strain
  * step is bracketed by HIP events on the grid's stream and its elapsed time

@@ -1,0 +1,127 @@
+"""Batched strain steps (vpfft_batch_*): many grids of uneven sizes advance
+together.  Each grid's result must be the same bits alone and in any batch,
+with or without block-shape tuning, and match the oracle."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from helpers import edot_tension, polycrystal, random_unit_edot
+
+pytestmark = pytest.mark.gpu
+
+# (dims, grains, n, precision)
+MIX = [((16, 16, 16), 8, 10, 0), ((24, 16, 20), 12, 20, 1), ((12, 18, 10), 5, 5, 0), ((32, 32, 32), 27, 20, 0),
+       ((20, 24, 14), 9, 10, 1)]
+
+
+def _grids(vpfft, cases, seed0=50, seeds=None):
+    out = []
+    for i, (dims, grains, nexp, prec) in enumerate(cases):
+        R, _ = polycrystal(None, grains, seed0 + (i if seeds is None else seeds[i]), dims)
+        g = vpfft.MaterialGrid(*dims, vpfft.Material.uniform(1.0, nexp), precision=prec)
+        g.SetVoxelLength(1.0, 1.0, 1.0)
+        g.SendDataAndInitialize(R)
+        out.append(g)
+    return out
+
+
+def _state(g):
+    return dict(sigma=g.Stress(), R=g.Orientation(), crss=g.CRSS(), lam=g.LagrangeMultiplier())
+
+
+def _run_batch(vpfft, cases, idx, steps, tune=False, max_iter=40, tol=1e-6):
+    grids = _grids(vpfft, [cases[i] for i in idx], seeds=idx)
+    edots = np.array([random_unit_edot(100 + i) for i in idx])
+    res = []
+    with vpfft.Batch(tune=tune) as b:
+        for s in range(steps):
+            avg = b.RunSingleStrainStep(grids, edots, max_iter, 0.005, tol)
+            res.append((avg.copy(), [inf.iterations for inf in b.last_infos]))
+        shapes = [[b.block_shape(g, p) for p in range(3)] for g in grids]
+    states = [_state(g) for g in grids]
+    for g in grids:
+        g.close()
+    return res, states, shapes
+
+
+def test_batch_results_do_not_depend_on_the_mix(gpu, vpfft):
+    print("batch of 5 uneven grids, 2 steps, vs each alone", flush=True)
+    together, st_all, _ = _run_batch(vpfft, MIX, list(range(len(MIX))), 2)
+    for j in range(len(MIX)):
+        alone, st_one, _ = _run_batch(vpfft, MIX, [j], 2)
+        for s in range(2):
+            assert together[s][1][j] == alone[s][1][0]
+            assert np.array_equal(together[s][0][j], alone[s][0][0])
+        for k in st_one[0]:
+            assert np.array_equal(st_all[j][k], st_one[0][k]), (j, k)
+
+
+def test_batch_tuned_block_shapes_give_the_same_bits(gpu, vpfft):
+    print("batch tuned vs default block shapes", flush=True)
+    plain, st_plain, shapes_plain = _run_batch(vpfft, MIX, [0, 1, 3], 1)
+    tuned, st_tuned, shapes_tuned = _run_batch(vpfft, MIX, [0, 1, 3], 1, tune=True)
+    assert np.array_equal(plain[0][0], tuned[0][0])
+    assert plain[0][1] == tuned[0][1]
+    for a, b in zip(st_plain, st_tuned):
+        for k in a:
+            assert np.array_equal(a[k], b[k])
+    print("default shapes", shapes_plain, "tuned", shapes_tuned, flush=True)
+
+
+def test_batch_of_one_matches_single_grid_path(gpu, vpfft):
+    print("batch of one vs vpfft_run_strain_step", flush=True)
+    dims = (24, 24, 24)
+    R, _ = polycrystal(None, 27, 9, dims)
+    mat = vpfft.Material.uniform(1.0, 20)
+    outs = []
+    for batched in (False, True):
+        with vpfft.MaterialGrid(*dims, mat) as g:
+            g.SetVoxelLength(1.0, 1.0, 1.0)
+            g.SendDataAndInitialize(R)
+            if batched:
+                with vpfft.Batch() as b:
+                    avg = b.RunSingleStrainStep([g], [edot_tension()], 30, 0.005, 1e-6)[0]
+            else:
+                avg = g.RunSingleStrainStep(edot_tension(), 30, 0.005, 1e-6)
+            outs.append((avg, g.last_info.iterations, g.Stress(), g.Orientation(), g.CRSS()))
+    # the reductions are grouped differently (rounding only)
+    assert outs[0][1] == outs[1][1]
+    assert rel_err(outs[1][0], outs[0][0]) < 1e-10
+    for k in (2, 3, 4):
+        assert rel_err(outs[1][k], outs[0][k]) < 1e-10
+
+
+def test_batch_matches_oracle(gpu, vpfft, oracle):
+    print("batch of 3 vs the oracle", flush=True)
+    cases = [((16, 16, 16), 8, 10, 0), ((12, 18, 10), 5, 5, 0), ((20, 16, 12), 6, 20, 0)]
+    grids = _grids(vpfft, cases, seed0=70)
+    edots = np.array([random_unit_edot(200 + j) for j in range(3)])
+    states = [dict(R=g.Orientation(), crss=g.CRSS(), acc=g.AccumulatedShear()) for g in grids]
+    with vpfft.Batch() as b:
+        avg = b.RunSingleStrainStep(grids, edots, 30, 0.005, 1e-6)
+        infos = b.last_infos
+    for j, (dims, grains, nexp, prec) in enumerate(cases):
+        ref = oracle.strain_step(dims, vpfft.Material.uniform(1.0, nexp), states[j]["R"], states[j]["crss"], edots[j],
+                                 30, 0.005, 1e-6, acc=states[j]["acc"])
+        assert infos[j].iterations == ref["iterations"]
+        assert rel_err(avg[j], ref["avg_stress"]) < 1e-8
+        assert rel_err(grids[j].Stress(), ref["sigma"]) < 1e-8
+        assert rel_err(grids[j].Orientation(), ref["R"]) < 1e-8
+        assert rel_err(grids[j].CRSS(), ref["crss"]) < 1e-8
+    for g in grids:
+        g.close()
+
+
+def test_batch_argument_errors(gpu, vpfft):
+    grids = _grids(vpfft, MIX[:2])
+    with vpfft.Batch() as b:
+        with pytest.raises(vpfft.VPFFTError, match="twice"):
+            b.RunSingleStrainStep([grids[0], grids[0]], [edot_tension()] * 2, 5, 0.005, 1e-6)
+        with pytest.raises(vpfft.VPFFTError, match="zero strain rate"):
+            b.RunSingleStrainStep(grids, [edot_tension(), np.zeros(5)], 5, 0.005, 1e-6)
+        fresh = vpfft.MaterialGrid(8, 8, 8, vpfft.Material.uniform(1.0, 10))
+        with pytest.raises(vpfft.VPFFTError, match="before upload"):
+            b.RunSingleStrainStep([grids[0], fresh], [edot_tension()] * 2, 5, 0.005, 1e-6)
+        fresh.close()
+    for g in grids:
+        g.close()

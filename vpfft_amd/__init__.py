@@ -122,10 +122,14 @@ EXPORTED = [
     "vpfft_run_vpfft", "vpfft_strain_rate_evolution", "vpfft_solve_constrained_batch",
     "vpfft_solve_taylor_batch", "vpfft_generate_voronoi", "vpfft_last_error", "vpfft_version",
     "vpfft_grid_set_profiling", "vpfft_grid_kernel_times", "vpfft_kernel_name",
+    "vpfft_batch_create", "vpfft_batch_destroy", "vpfft_batch_set_tuning", "vpfft_batch_block_shape",
+    "vpfft_batch_run_strain_step", "vpfft_grid_bytes", "vpfft_grid_save_state", "vpfft_grid_restore_state",
     # include/vpfft_io.h
     "vpfft_deck_read", "vpfft_deck_parse", "vpfft_deck_material", "vpfft_deck_strain_rate",
     "vpfft_two_grain_init", "vpfft_euler_deg_to_matrix", "vpfft_read_euler_file", "vpfft_write_euler_file",
     "vpfft_write_material_file", "vpfft_read_material_file", "vpfft_run_deck",
+    # include/vpfft_serve.h
+    "vpfft_serve",
 ]
 
 
@@ -162,6 +166,15 @@ def load_library(path: Optional[str] = None):
     lib.vpfft_kernel_name.argtypes = [C.c_int]
     ip = C.POINTER(C.c_int32)
     cp = C.c_char_p
+    lib.vpfft_batch_create.argtypes = [C.POINTER(vp)]
+    lib.vpfft_batch_destroy.argtypes = [vp]
+    lib.vpfft_batch_set_tuning.argtypes = [vp, C.c_int]
+    lib.vpfft_batch_block_shape.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
+    lib.vpfft_batch_run_strain_step.argtypes = [vp, C.POINTER(vp), C.c_int, dp, C.c_int, dp, C.c_double,
+                                                dp, C.POINTER(vpfft_step_info)]
+    lib.vpfft_grid_bytes.argtypes = [C.POINTER(vpfft_grid_desc), C.POINTER(C.c_uint64)]
+    lib.vpfft_grid_save_state.argtypes = [vp]
+    lib.vpfft_grid_restore_state.argtypes = [vp]
     lib.vpfft_deck_read.argtypes = [cp, C.POINTER(vpfft_deck)]
     lib.vpfft_deck_parse.argtypes = [cp, C.POINTER(vpfft_deck)]
     lib.vpfft_deck_material.argtypes = [C.POINTER(vpfft_deck), C.POINTER(vpfft_material)]
@@ -481,3 +494,59 @@ def solve_taylor_batch(material: Material, R, crss, edot, stress):
     _check(lib.vpfft_solve_taylor_batch(n, C.byref(m), _dptr(R), _dptr(crss), _dptr(edot), _dptr(stress),
                                         _dptr(out)))
     return out
+
+
+# ---------------------------------------------------------------------------
+# batched strain steps
+# ---------------------------------------------------------------------------
+class Batch:
+    """Advance several MaterialGrids (any sizes) through one strain step each,
+    together (include/vpfft.h, vpfft_batch_*)."""
+
+    def __init__(self, tune: bool = False):
+        self._lib = load_library()
+        h = C.c_void_p()
+        _check(self._lib.vpfft_batch_create(C.byref(h)))
+        self._h = h
+        if tune:
+            _check(self._lib.vpfft_batch_set_tuning(self._h, 1))
+        self.last_infos: list = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.vpfft_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def block_shape(self, grid: "MaterialGrid", pass_: int) -> int:
+        b = C.c_int()
+        _check(self._lib.vpfft_batch_block_shape(self._h, grid._h, pass_, C.byref(b)))
+        return b.value
+
+    def RunSingleStrainStep(self, grids, edots, NumMaxIterations: int, TimeStep: float,
+                            ConvergenceEpsilon: float) -> np.ndarray:
+        """One strain step of every grid; returns <sigma> per grid (count x 5)."""
+        count = len(grids)
+        e = _f64(edots, (count, 5))
+        dts = _f64(np.broadcast_to(np.asarray(TimeStep, dtype=np.float64), (count,)))
+        arr = (C.c_void_p * count)(*[g._h for g in grids])
+        avg = np.empty((count, 5))
+        infos = (vpfft_step_info * count)()
+        rc = self._lib.vpfft_batch_run_strain_step(self._h, arr, count, _dptr(e), int(NumMaxIterations),
+                                                    _dptr(dts), float(ConvergenceEpsilon), _dptr(avg), infos)
+        self.last_infos = [infos[j] for j in range(count)]
+        for g, inf in zip(grids, self.last_infos):
+            g.last_info = inf
+        _check(rc)
+        return avg

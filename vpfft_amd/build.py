@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvpfft.so")
 RUNNER = os.path.join(HERE, "vpfft_run")
-SOURCES = ["capi.hip", "fft_green.hip", "voxel_kernels.hip", "microstructure.cpp", "deck_io.cpp"]
+SOURCES = ["capi.hip", "fft_green.hip", "voxel_kernels.hip", "microstructure.cpp", "deck_io.cpp", "serve.cpp"]
 ARCH = os.environ.get("VPFFT_OFFLOAD_ARCH", "gfx950")
 
 

@@ -9,7 +9,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <utility>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/vpfft.h"
@@ -67,6 +70,7 @@ static const char* kKernelNames[kNumKernels] = {"prologue", "L0_finalize", "fft_
                                                 "iter_finalize", "epilogue"};
 
 struct Grid {
+  unsigned long serial = 0;  // unique per created grid (keys of captured batch graphs)
   int nx = 0, ny = 0, nz = 0, nzh = 0, nzp = 0;
   long n = 0, nc = 0;
   int prec = 0;
@@ -112,6 +116,10 @@ struct Grid {
     }
   } gkey{-1, 0, 0, 0, 0.0};
   int phi_prec = 0, dtil_prec = 0;  // precision the field buffers currently hold
+  // device copy of the state a strain step changes (vpfft_grid_save_state)
+  double* saved = nullptr;
+  double saved_len[3] = {1, 1, 1};
+  bool saved_crss_uniform = false, has_saved = false;
   // per-kernel timing with HIP events on the grid's stream
   bool profile = false;
   hipEvent_t kev[kNumKernels][2] = {};
@@ -344,6 +352,20 @@ static int check_material(const vpfft_material* m) {
   return VPFFT_OK;
 }
 
+// Device bytes grid_create allocates for these dimensions (records at their
+// initial capacity; the saved-state copy not included).
+static size_t grid_bytes(const vpfft_grid_desc* d) {
+  const long n = (long)d->nx * d->ny * d->nz;
+  const long nc = (long)d->nx * d->ny * padded_half(d->nz);
+  size_t b = sizeof(double) * 42 * (size_t)n + sizeof(double2) * 5 * (size_t)nc;
+  b += sizeof(StepConst) + sizeof(GreenConst<double>) + sizeof(GreenConst<float>) + sizeof(PrologueRecord) +
+       sizeof(LoopState) + sizeof(IterRecord) * 128;
+  b += sizeof(double) * (size_t)std::max<long>(4096L * 18, nr_partial_rows(n) * kPartialStride);
+  b += sizeof(double) * (size_t)std::min<long>(60L * n, 8L << 20);
+  b += (sizeof(double2) + sizeof(float2)) * (size_t)(d->nx + d->ny + d->nz);
+  return b;
+}
+
 static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid** out) {
   if (!d || !mat || !out) return set_error(VPFFT_E_INVALID, "null argument");
   *out = nullptr;
@@ -353,7 +375,9 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
     return set_error(VPFFT_E_INVALID, "precision must be VPFFT_FP64 or VPFFT_MIXED");
   int rc = check_material(mat);
   if (rc) return rc;
+  static unsigned long next_serial = 0;
   Grid* g = new Grid();
+  g->serial = ++next_serial;
   g->nx = d->nx;
   g->ny = d->ny;
   g->nz = d->nz;
@@ -609,6 +633,468 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
     info->status = finite ? VPFFT_OK : VPFFT_E_NONFINITE;
   }
   if (!finite) return set_error(VPFFT_E_NONFINITE, "non-finite average stress or error norm");
+  return VPFFT_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched strain steps.  Many grids (independent RVEs of any size) advance
+// through one strain step together: every pass of the fixed-point loop is one
+// launch per group of grids that share a kernel variant (precision, FFT block
+// shape, Newton specialisation), so an iteration of the whole batch is a
+// handful of launches however many grids it holds, and the iteration is
+// replayed as a HIP graph.  Each grid keeps its own reference medium, loop
+// flag and iteration count; a converged grid's blocks return at entry.  The
+// FFT block shapes come from the defaults or from a tuner that times the
+// candidates per (precision, pass, grid shape) once; a block shape only
+// changes how pencils are grouped into blocks, never the arithmetic, so every
+// grid's result is the same bits alone and in any batch.
+// ---------------------------------------------------------------------------
+enum LaunchKind { L_PRO, L_ZF, L_YF, L_X, L_YI, L_ZI, L_NR, L_EPI };
+
+struct BLaunch {
+  int kind, prec, B, E, nexp, uniform;
+  long blocks;
+  size_t lds;
+  int ids_at, off_at, count;  // slices of the id / offset pools
+};
+
+struct Batch {
+  hipStream_t st = nullptr;
+  int cap = 0;
+  FftJob* dfft = nullptr;
+  VoxJob* dvox = nullptr;
+  LoopState* dls = nullptr;
+  int* dids = nullptr;
+  long* doff = nullptr;
+  int pool_cap = 0;
+  // pinned host mirrors
+  FftJob* hfft = nullptr;
+  VoxJob* hvox = nullptr;
+  LoopState* hls = nullptr;
+  int* hids = nullptr;
+  long* hoff = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int graph_iters = 4;
+  int tune = 0;
+  std::map<std::tuple<int, int, int, int, int>, int> tuned;  // (prec, pass, nx, ny, nz) -> B
+  hipGraphExec_t gexec = nullptr;
+  std::vector<unsigned long> gkey_grids;
+  std::vector<int> gkey_vals;
+  double gkey_tol = 0;
+  std::vector<BLaunch> launches;
+};
+
+static void batch_free_tables(Batch* b) {
+  for (void* p : {(void*)b->dfft, (void*)b->dvox, (void*)b->dls, (void*)b->dids, (void*)b->doff})
+    if (p) (void)hipFree(p);
+  for (void* p : {(void*)b->hfft, (void*)b->hvox, (void*)b->hls, (void*)b->hids, (void*)b->hoff})
+    if (p) (void)hipHostFree(p);
+  b->dfft = nullptr; b->dvox = nullptr; b->dls = nullptr; b->dids = nullptr; b->doff = nullptr;
+  b->hfft = nullptr; b->hvox = nullptr; b->hls = nullptr; b->hids = nullptr; b->hoff = nullptr;
+  b->cap = b->pool_cap = 0;
+}
+
+static void batch_destroy(Batch* b) {
+  if (!b) return;
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  batch_free_tables(b);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+}
+
+static int batch_reserve(Batch* b, int count) {
+  const int pool = 16 * count + 64;  // ids and offsets of every launch of one step
+  if (count <= b->cap && pool <= b->pool_cap) return VPFFT_OK;
+  if (b->gexec) {
+    (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+    b->gkey_grids.clear();
+  }
+  batch_free_tables(b);
+  const int cap = std::max(count, 8);
+  const int pcap = 16 * cap + 64;
+  auto dal = [](void** p, size_t bytes) { return hipMalloc(p, bytes) == hipSuccess; };
+  auto hal = [](void** p, size_t bytes) { return hipHostMalloc(p, bytes, hipHostMallocDefault) == hipSuccess; };
+  bool ok = dal((void**)&b->dfft, sizeof(FftJob) * cap) && dal((void**)&b->dvox, sizeof(VoxJob) * cap) &&
+            dal((void**)&b->dls, sizeof(LoopState) * cap) && dal((void**)&b->dids, sizeof(int) * pcap) &&
+            dal((void**)&b->doff, sizeof(long) * pcap) && hal((void**)&b->hfft, sizeof(FftJob) * cap) &&
+            hal((void**)&b->hvox, sizeof(VoxJob) * cap) && hal((void**)&b->hls, sizeof(LoopState) * cap) &&
+            hal((void**)&b->hids, sizeof(int) * pcap) && hal((void**)&b->hoff, sizeof(long) * pcap);
+  if (!ok) {
+    batch_free_tables(b);
+    return set_error(VPFFT_E_NOMEM, "batch tables for %d grids", count);
+  }
+  b->cap = cap;
+  b->pool_cap = pcap;
+  return VPFFT_OK;
+}
+
+static void fill_fft_job(const Grid* g, const int* done, FftJob* J) {
+  const int prec = g->prec;
+  J->g = g->geom;
+  J->ax = g->plan[0];
+  J->ay = g->plan[1];
+  J->az = g->plan[2];
+  for (int a = 0; a < 3; a++) J->tw[a] = prec == 0 ? g->tw64[a] : g->tw32[a];
+  J->field = g->phi;
+  J->dtil = g->dtil;
+  J->spec = g->spec;
+  J->gc = prec == 0 ? (const void*)g->dG64 : (const void*)g->dG32;
+  J->done = done;
+}
+
+static void fill_vox_job(Grid* g, LoopState* ls, VoxJob* J) {
+  J->n = g->n;
+  J->dtil = g->dtil;
+  J->phi = g->phi;
+  J->sigma = g->sigma;
+  J->lambda = g->lambda;
+  J->R = g->R;
+  J->crss = g->crss;
+  J->acc = g->acc;
+  J->C = g->dC;
+  J->g64 = g->dG64;
+  J->g32 = g->dG32;
+  J->partial = g->partial;
+  J->ls = ls;
+  J->rec = g->drec;
+  J->prol = g->dprol;
+}
+
+static long pass_blocks(const Grid* g, int kind, int B) {
+  if (kind == L_PRO || kind == L_NR || kind == L_EPI) return nr_blocks(g->n);
+  const FftPass pass = (kind == L_ZF || kind == L_ZI) ? kPassZ : ((kind == L_YF || kind == L_YI) ? kPassY : kPassX);
+  return pass_shape(g->prec, pass, g->geom, B).blocks;
+}
+
+// One grouped launch over the batch's FFT job table.
+static int enqueue_fft_launch(Batch* b, const BLaunch& L) {
+  const FftGroup grp{b->dfft, b->dids + L.ids_at, b->doff + L.off_at, L.count};
+  switch (L.kind) {
+    case L_ZF: HIP_TRY(launch_z_r2c_group(L.prec, L.B, grp, L.blocks, L.lds, b->st)); break;
+    case L_YF: HIP_TRY(launch_y_group(L.prec, L.B, grp, L.blocks, L.lds, -1, b->st)); break;
+    case L_X: HIP_TRY(launch_x_green_group(L.prec, L.B, L.E, grp, L.blocks, L.lds, b->st)); break;
+    case L_YI: HIP_TRY(launch_y_group(L.prec, L.B, grp, L.blocks, L.lds, +1, b->st)); break;
+    case L_ZI: HIP_TRY(launch_z_c2r_group(L.prec, L.B, grp, L.blocks, L.lds, b->st)); break;
+  }
+  return VPFFT_OK;
+}
+
+// Block shape of an FFT pass for grid g: the tuned one when tuning is on.
+static int choose_B(Batch* b, Grid* g, FftPass pass, int job) {
+  if (!b->tune) return pass_shape(g->prec, pass, g->geom, 0).B;
+  const auto key = std::make_tuple(g->prec, (int)pass, g->nx, g->ny, g->nz);
+  auto it = b->tuned.find(key);
+  if (it != b->tuned.end()) return it->second;
+  int cand[8];
+  const int nc = pass_candidates(pass, cand, 8);
+  int best = pass_shape(g->prec, pass, g->geom, 0).B;
+  float best_ms = 1e30f;
+  // a one-job group at the end of the pools (the step's launches use the front)
+  const int at = b->pool_cap - 4;
+  b->hids[at] = job;
+  b->hoff[at + 1] = 0;
+  for (int c = 0; c < nc; c++) {
+    const PassShape ps = pass_shape(g->prec, pass, g->geom, cand[c]);
+    if (!ps.ok) continue;
+    b->hoff[at + 2] = ps.blocks;
+    if (hipMemcpyAsync(b->dids + at, b->hids + at, sizeof(int), hipMemcpyHostToDevice, b->st) != hipSuccess ||
+        hipMemcpyAsync(b->doff + at + 1, b->hoff + at + 1, 2 * sizeof(long), hipMemcpyHostToDevice, b->st) !=
+            hipSuccess)
+      return -1;
+    BLaunch L{pass == kPassZ ? (int)L_ZF : (pass == kPassY ? (int)L_YF : (int)L_X), g->prec, ps.B, ps.E, 0, 0,
+              ps.blocks, ps.lds, at, at + 1, 1};
+    float ms = 0;
+    for (int rep = 0; rep < 4; rep++) {
+      if (rep == 1 && hipEventRecord(b->ev[0], b->st) != hipSuccess) return -1;
+      if (enqueue_fft_launch(b, L)) return -1;
+    }
+    if (hipEventRecord(b->ev[1], b->st) != hipSuccess || hipEventSynchronize(b->ev[1]) != hipSuccess) return -1;
+    (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = ps.B;
+    }
+  }
+  b->tuned[key] = best;
+  return best;
+}
+
+// Group the grids of a step into launches and upload the tables.
+static int batch_plan(Batch* b, Grid** gs, int count, const double* dt) {
+  b->launches.clear();
+  int ids_at = 0, off_at = 0;
+  auto add_group = [&](int kind, int prec, int B, int E, int nexp, int uniform, const std::vector<int>& members) {
+    BLaunch L{kind, prec, B, E, nexp, uniform, 0, 0, ids_at, off_at, (int)members.size()};
+    b->hoff[off_at] = 0;
+    for (size_t m = 0; m < members.size(); m++) {
+      Grid* g = gs[members[m]];
+      b->hids[ids_at + m] = members[m];
+      L.blocks += pass_blocks(g, kind, B);
+      b->hoff[off_at + m + 1] = L.blocks;
+      if (kind != L_PRO && kind != L_NR && kind != L_EPI) {
+        const FftPass pass = (kind == L_ZF || kind == L_ZI) ? kPassZ : ((kind == L_YF || kind == L_YI) ? kPassY : kPassX);
+        L.lds = std::max(L.lds, pass_shape(g->prec, pass, g->geom, B).lds);
+      }
+    }
+    ids_at += (int)members.size();
+    off_at += (int)members.size() + 1;
+    b->launches.push_back(L);
+  };
+  // NR / epilogue groups (the prologue ran before the plan)
+  std::map<std::tuple<int, int, int>, std::vector<int>> nr;
+  std::vector<int> epi;
+  for (int j = 0; j < count; j++) {
+    nr[std::make_tuple(gs[j]->prec, gs[j]->nexp_uniform, gs[j]->crss_uniform ? 1 : 0)].push_back(j);
+    if (dt[j] > 0) epi.push_back(j);
+  }
+  // FFT passes: groups by (precision, block shape)
+  const int kinds[5] = {L_ZF, L_YF, L_X, L_YI, L_ZI};
+  for (int kind : kinds) {
+    const FftPass pass = (kind == L_ZF || kind == L_ZI) ? kPassZ : ((kind == L_YF || kind == L_YI) ? kPassY : kPassX);
+    std::map<std::tuple<int, int, int>, std::vector<int>> fg;
+    for (int j = 0; j < count; j++) {
+      const int B = choose_B(b, gs[j], pass, j);
+      if (B < 0) return set_error(VPFFT_E_DEVICE, "FFT block-shape tuning failed");
+      const PassShape ps = pass_shape(gs[j]->prec, pass, gs[j]->geom, B);
+      fg[std::make_tuple(gs[j]->prec, ps.B, ps.E)].push_back(j);
+    }
+    for (auto& kv : fg) add_group(kind, std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), 0, 0,
+                                  kv.second);
+  }
+  for (auto& kv : nr)
+    add_group(L_NR, std::get<0>(kv.first), 0, 0, std::get<1>(kv.first), std::get<2>(kv.first), kv.second);
+  // epilogue for the grids that move in time (MaterialGrid.cpp:1074-1079)
+  if (!epi.empty()) add_group(L_EPI, 0, 0, 0, 0, 0, epi);
+  // the one-block-per-grid kernels use the identity list
+  for (int j = 0; j < count; j++) b->hids[ids_at + j] = j;
+  b->launches.push_back(BLaunch{-1, 0, 0, 0, 0, 0, 0, 0, ids_at, off_at, count});
+  ids_at += count;
+  if (ids_at > b->pool_cap - 8 || off_at > b->pool_cap - 8) return set_error(VPFFT_E_INVALID, "batch plan overflow");
+  HIP_TRY(hipMemcpyAsync(b->dids, b->hids, sizeof(int) * ids_at, hipMemcpyHostToDevice, b->st));
+  HIP_TRY(hipMemcpyAsync(b->doff, b->hoff, sizeof(long) * std::max(off_at, 1), hipMemcpyHostToDevice, b->st));
+  return VPFFT_OK;
+}
+
+static VoxGroup vox_group(Batch* b, const BLaunch& L) {
+  return VoxGroup{b->dvox, b->dids + L.ids_at, b->doff + L.off_at, L.count};
+}
+
+static int batch_iteration(Batch* b, double tol, int max_iter) {
+  const BLaunch& all = b->launches.back();
+  const int order[5] = {L_ZF, L_YF, L_X, L_YI, L_ZI};
+  for (int kind : order)
+    for (const BLaunch& L : b->launches)
+      if (L.kind == kind) {
+        int rc = enqueue_fft_launch(b, L);
+        if (rc) return rc;
+      }
+  for (const BLaunch& L : b->launches)
+    if (L.kind == L_NR) HIP_TRY(launch_nr_group(L.prec, L.nexp, L.uniform != 0, vox_group(b, L), L.blocks, b->st));
+  HIP_TRY(launch_iter_finalize_group(vox_group(b, all), tol, max_iter, b->st));
+  return VPFFT_OK;
+}
+
+static int batch_step(Batch* b, Grid** gs, int count, const double* edot, int max_iter, const double* dt,
+                      double tol, double* avg, vpfft_step_info* infos) {
+  if (count < 1 || !gs || !edot || !dt) return set_error(VPFFT_E_INVALID, "empty batch");
+  if (max_iter < 1) return set_error(VPFFT_E_INVALID, "max_iter must be >= 1");
+  if (!(tol >= 0)) return set_error(VPFFT_E_INVALID, "bad tolerance");
+  for (int j = 0; j < count; j++) {
+    if (!gs[j]) return set_error(VPFFT_E_INVALID, "null grid %d", j);
+    if (!std::isfinite(dt[j])) return set_error(VPFFT_E_INVALID, "grid %d: bad time step", j);
+    if (!gs[j]->uploaded) return set_error(VPFFT_E_STATE, "grid %d: run before upload", j);
+    for (int k = 0; k < j; k++)
+      if (gs[k] == gs[j]) return set_error(VPFFT_E_INVALID, "grid %d appears twice", j);
+    double ee = 0;
+    for (int i = 0; i < 5; i++) {
+      if (!std::isfinite(edot[5 * j + i])) return set_error(VPFFT_E_NONFINITE, "grid %d: non-finite strain rate", j);
+      ee += edot[5 * j + i] * edot[5 * j + i];
+    }
+    if (!(ee > 0)) return set_error(VPFFT_E_DEGENERATE, "grid %d: zero strain rate", j);
+  }
+  int rc = batch_reserve(b, count);
+  if (rc) return rc;
+  for (int j = 0; j < count; j++) {
+    Grid* g = gs[j];
+    if ((rc = ensure_records(g, max_iter))) return rc;
+    fill_host_const(g, edot + 5 * j, dt[j]);
+    HIP_TRY(hipMemcpyAsync(g->dC, g->hC, sizeof(StepConst), hipMemcpyHostToDevice, b->st));
+    fill_vox_job(g, b->dls + j, b->hvox + j);
+    fill_fft_job(g, &b->dls[j].done, b->hfft + j);
+  }
+  HIP_TRY(hipMemcpyAsync(b->dvox, b->hvox, sizeof(VoxJob) * count, hipMemcpyHostToDevice, b->st));
+  HIP_TRY(hipMemcpyAsync(b->dfft, b->hfft, sizeof(FftJob) * count, hipMemcpyHostToDevice, b->st));
+  // prologue first (the tuner times the FFT passes on valid Green constants)
+  {
+    std::map<int, std::vector<int>> pro;
+    for (int j = 0; j < count; j++) pro[gs[j]->prec].push_back(j);
+    int ids_at = 0, off_at = 0;
+    std::vector<BLaunch> pl;
+    for (auto& kv : pro) {
+      BLaunch L{L_PRO, kv.first, 0, 0, 0, 0, 0, 0, ids_at, off_at, (int)kv.second.size()};
+      b->hoff[off_at] = 0;
+      for (size_t m = 0; m < kv.second.size(); m++) {
+        b->hids[ids_at + m] = kv.second[m];
+        L.blocks += nr_blocks(gs[kv.second[m]]->n);
+        b->hoff[off_at + m + 1] = L.blocks;
+      }
+      ids_at += L.count;
+      off_at += L.count + 1;
+      pl.push_back(L);
+    }
+    for (int j = 0; j < count; j++) b->hids[ids_at + j] = j;
+    HIP_TRY(hipMemcpyAsync(b->dids, b->hids, sizeof(int) * (ids_at + count), hipMemcpyHostToDevice, b->st));
+    HIP_TRY(hipMemcpyAsync(b->doff, b->hoff, sizeof(long) * off_at, hipMemcpyHostToDevice, b->st));
+    HIP_TRY(hipEventRecord(b->ev[0], b->st));
+    for (const BLaunch& L : pl) HIP_TRY(launch_prologue_group(L.prec, vox_group(b, L), L.blocks, b->st));
+    const VoxGroup all{b->dvox, b->dids + ids_at, nullptr, count};
+    HIP_TRY(launch_L0_finalize_group(all, b->st));
+    HIP_TRY(launch_loop_reset_group(all, b->st));
+    for (int j = 0; j < count; j++)
+      HIP_TRY(hipMemcpyAsync(gs[j]->hprol, gs[j]->dprol, sizeof(PrologueRecord), hipMemcpyDeviceToHost, b->st));
+    HIP_TRY(hipEventRecord(b->ev[1], b->st));
+    HIP_TRY(hipStreamSynchronize(b->st));
+    for (int j = 0; j < count; j++)
+      if (gs[j]->hprol->degenerate > 0)
+        return set_error(VPFFT_E_DEGENERATE, "grid %d: %.0f voxels with max resolved shear stress < 1e-10", j,
+                         gs[j]->hprol->degenerate);
+  }
+  if ((rc = batch_plan(b, gs, count, dt))) return rc;
+  // the loop, replayed as a graph of graph_iters iterations
+  std::vector<unsigned long> key_grids(count);
+  for (int j = 0; j < count; j++) key_grids[j] = gs[j]->serial;
+  std::vector<int> key_vals{max_iter, count};
+  for (const BLaunch& L : b->launches) {
+    key_vals.push_back(L.kind);
+    key_vals.push_back(L.B);
+    key_vals.push_back((int)L.blocks);
+  }
+  for (int j = 0; j < count; j++) {
+    key_vals.push_back(gs[j]->prec);
+    key_vals.push_back(gs[j]->crss_uniform);
+  }
+  if (b->graph_iters > 0) {
+    if (!b->gexec || key_grids != b->gkey_grids || key_vals != b->gkey_vals || tol != b->gkey_tol) {
+      if (b->gexec) {
+        HIP_TRY(hipGraphExecDestroy(b->gexec));
+        b->gexec = nullptr;
+      }
+      hipGraph_t graph;
+      HIP_TRY(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+      int crc = VPFFT_OK;
+      for (int k = 0; k < b->graph_iters && crc == VPFFT_OK; k++) crc = batch_iteration(b, tol, max_iter);
+      hipError_t ce = hipStreamEndCapture(b->st, &graph);
+      if (crc) return crc;
+      HIP_TRY(ce);
+      hipError_t ie = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIP_TRY(ie);
+      b->gkey_grids = key_grids;
+      b->gkey_vals = key_vals;
+      b->gkey_tol = tol;
+    }
+  }
+  for (;;) {
+    if (b->graph_iters > 0) {
+      HIP_TRY(hipGraphLaunch(b->gexec, b->st));
+    } else if ((rc = batch_iteration(b, tol, max_iter))) {
+      return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(b->hls, b->dls, sizeof(LoopState) * count, hipMemcpyDeviceToHost, b->st));
+    HIP_TRY(hipStreamSynchronize(b->st));
+    bool all_done = true;
+    for (int j = 0; j < count; j++) all_done = all_done && b->hls[j].done;
+    if (all_done) break;
+  }
+  std::vector<int> iters(count);
+  for (int j = 0; j < count; j++) {
+    iters[j] = b->hls[j].iter;
+    HIP_TRY(hipMemcpyAsync(gs[j]->hrec, gs[j]->drec, sizeof(IterRecord) * iters[j], hipMemcpyDeviceToHost, b->st));
+  }
+  HIP_TRY(hipEventRecord(b->ev[2], b->st));
+  for (const BLaunch& L : b->launches)
+    if (L.kind == L_EPI) HIP_TRY(launch_epilogue_group(vox_group(b, L), L.blocks, b->st));
+  HIP_TRY(hipEventRecord(b->ev[3], b->st));
+  HIP_TRY(hipStreamSynchronize(b->st));
+  float ms[3] = {0, 0, 0};
+  for (int k = 0; k < 3; k++) (void)hipEventElapsedTime(&ms[k], b->ev[k], b->ev[k + 1]);
+  int status = VPFFT_OK;
+  for (int j = 0; j < count; j++) {
+    Grid* g = gs[j];
+    g->phi_prec = g->dtil_prec = g->prec;
+    const int it = iters[j];
+    double nr_total = 0, wave_total = 0;
+    int degen_at = 0;
+    for (int i = 0; i < it; i++) {
+      nr_total += g->hrec[i].nr_steps;
+      wave_total += g->hrec[i].nr_wave_steps;
+      if (!degen_at && g->hrec[i].degenerate > 0) degen_at = i + 1;
+    }
+    if (dt[j] > 0) {
+      g->crss_uniform = g->crss_uniform && g->hardening_uniform;
+      double d[3];
+      edot_diag(edot + 5 * j, d);
+      g->lx *= (1.0 + d[0] * dt[j]);
+      g->ly *= (1.0 + d[1] * dt[j]);
+      g->lz *= (1.0 + d[2] * dt[j]);
+    }
+    const IterRecord& last = g->hrec[it - 1];
+    bool finite = true;
+    for (int i = 0; i < 5; i++) finite = finite && std::isfinite(last.avg_stress[i]);
+    finite = finite && std::isfinite(last.err_stress) && std::isfinite(last.err_strain_rate);
+    const int st = degen_at ? VPFFT_E_DEGENERATE : (finite ? VPFFT_OK : VPFFT_E_NONFINITE);
+    if (st && !status) {
+      status = st;
+      set_error(st, degen_at ? "grid %d: degenerate voxels (iteration %d)" : "grid %d: non-finite result", j,
+                degen_at);
+    }
+    if (avg)
+      for (int i = 0; i < 5; i++) avg[5 * j + i] = last.avg_stress[i];
+    if (infos) {
+      vpfft_step_info* info = infos + j;
+      std::memset(info, 0, sizeof *info);
+      info->iterations = it;
+      info->converged = (last.err_stress <= tol && last.err_strain_rate <= tol) ? 1 : 0;
+      info->status = st;
+      info->precision = g->prec;
+      info->err_stress = last.err_stress;
+      info->err_strain_rate = last.err_strain_rate;
+      for (int i = 0; i < 5; i++) {
+        info->avg_stress[i] = last.avg_stress[i];
+        info->avg_strain_rate[i] = last.avg_strain_rate[i];
+      }
+      for (int e = 0; e < 25; e++) info->L0[e] = g->hprol->L0[e];
+      info->max_nr_error = last.max_nr_error;
+      info->max_nr_steps = last.max_nr_steps;
+      info->nr_steps_last = last.nr_steps;
+      info->nr_steps_total = nr_total;
+      info->nr_wave_steps_total = wave_total;
+      info->ms_prologue = ms[0];
+      info->ms_loop = ms[1];
+      info->ms_epilogue = ms[2];
+    }
+  }
+  return status;
+}
+
+static int batch_create(Batch** out) {
+  Batch* b = new Batch();
+  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return set_error(VPFFT_E_DEVICE, "hipStreamCreate failed");
+  }
+  for (auto& e : b->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      batch_destroy(b);
+      return set_error(VPFFT_E_DEVICE, "hipEventCreate failed");
+    }
+  if (const char* k = std::getenv("VPFFT_GRAPH_ITERS")) b->graph_iters = std::max(0, std::min(64, std::atoi(k)));
+  *out = b;
   return VPFFT_OK;
 }
 
@@ -942,4 +1428,99 @@ int vpfft_solve_taylor_batch(long n, const vpfft_material* mat, const double* or
   return VPFFT_OK;
 }
 
+int vpfft_batch_create(void** out_batch) {
+  if (!out_batch) return set_error(VPFFT_E_INVALID, "null argument");
+  vpfft::Batch* b = nullptr;
+  int rc = vpfft::batch_create(&b);
+  *out_batch = rc == VPFFT_OK ? (void*)b : nullptr;
+  return rc;
+}
+
+int vpfft_batch_destroy(void* batch) {
+  vpfft::batch_destroy(static_cast<vpfft::Batch*>(batch));
+  return VPFFT_OK;
+}
+
+int vpfft_batch_set_tuning(void* batch, int enable) {
+  if (!batch) return set_error(VPFFT_E_INVALID, "null batch");
+  static_cast<vpfft::Batch*>(batch)->tune = enable ? 1 : 0;
+  return VPFFT_OK;
+}
+
+int vpfft_batch_block_shape(void* batch, const void* grid, int pass, int* block_out) {
+  vpfft::Batch* b = static_cast<vpfft::Batch*>(batch);
+  const Grid* g = static_cast<const Grid*>(grid);
+  if (!b || !g || !block_out || pass < 0 || pass > 2) return set_error(VPFFT_E_INVALID, "bad argument");
+  const auto key = std::make_tuple(g->prec, pass, g->nx, g->ny, g->nz);
+  auto it = b->tuned.find(key);
+  *block_out = (b->tune && it != b->tuned.end()) ? it->second
+                                                 : vpfft::pass_shape(g->prec, (vpfft::FftPass)pass, g->geom, 0).B;
+  return VPFFT_OK;
+}
+
+int vpfft_batch_run_strain_step(void* batch, void* const* grids, int count, const double* edot, int max_iter,
+                                const double* dt, double tol, double* avg_stress_out, vpfft_step_info* infos) {
+  vpfft::Batch* b = static_cast<vpfft::Batch*>(batch);
+  if (!b) return set_error(VPFFT_E_INVALID, "null batch");
+  std::vector<Grid*> gs(count > 0 ? count : 0);
+  for (int j = 0; j < count; j++) gs[j] = static_cast<Grid*>(grids[j]);
+  return vpfft::batch_step(b, gs.data(), count, edot, max_iter, dt, tol, avg_stress_out, infos);
+}
+
+int vpfft_grid_bytes(const vpfft_grid_desc* desc, uint64_t* bytes_out) {
+  if (!desc || !bytes_out || desc->nx < 2 || desc->ny < 2 || desc->nz < 2)
+    return set_error(VPFFT_E_INVALID, "bad argument");
+  *bytes_out = (uint64_t)vpfft::grid_bytes(desc);
+  return VPFFT_OK;
+}
+
+int vpfft_grid_save_state(void* grid) {
+  Grid* g = as_grid(grid);
+  if (!g) return set_error(VPFFT_E_INVALID, "null grid");
+  if (!g->uploaded) return set_error(VPFFT_E_STATE, "save_state before upload");
+  const long n = g->n;
+  if (!g->saved) {
+    void* p = nullptr;
+    int rc = vpfft::dalloc(g, &p, sizeof(double) * 32 * n);
+    if (rc) return rc;
+    g->saved = (double*)p;
+  }
+  double* d = g->saved;
+  const std::pair<double*, long> parts[5] = {{g->R, 9}, {g->crss, 12}, {g->acc, 1}, {g->sigma, 5}, {g->lambda, 5}};
+  for (const auto& pr : parts) {
+    if (hipMemcpyAsync(d, pr.first, sizeof(double) * pr.second * n, hipMemcpyDeviceToDevice, g->st) != hipSuccess)
+      return set_error(VPFFT_E_DEVICE, "save_state copy failed");
+    d += pr.second * n;
+  }
+  if (hipStreamSynchronize(g->st) != hipSuccess) return set_error(VPFFT_E_DEVICE, "save_state sync failed");
+  g->saved_len[0] = g->lx;
+  g->saved_len[1] = g->ly;
+  g->saved_len[2] = g->lz;
+  g->saved_crss_uniform = g->crss_uniform;
+  g->has_saved = true;
+  return VPFFT_OK;
+}
+
+int vpfft_grid_restore_state(void* grid) {
+  Grid* g = as_grid(grid);
+  if (!g) return set_error(VPFFT_E_INVALID, "null grid");
+  if (!g->has_saved) return set_error(VPFFT_E_STATE, "restore_state without a saved state");
+  const long n = g->n;
+  const double* d = g->saved;
+  const std::pair<double*, long> parts[5] = {{g->R, 9}, {g->crss, 12}, {g->acc, 1}, {g->sigma, 5}, {g->lambda, 5}};
+  for (const auto& pr : parts) {
+    if (hipMemcpyAsync(pr.first, d, sizeof(double) * pr.second * n, hipMemcpyDeviceToDevice, g->st) != hipSuccess)
+      return set_error(VPFFT_E_DEVICE, "restore_state copy failed");
+    d += pr.second * n;
+  }
+  if (hipStreamSynchronize(g->st) != hipSuccess) return set_error(VPFFT_E_DEVICE, "restore_state sync failed");
+  g->lx = g->saved_len[0];
+  g->ly = g->saved_len[1];
+  g->lz = g->saved_len[2];
+  g->crss_uniform = g->saved_crss_uniform;
+  return VPFFT_OK;
+}
+
 }  // extern "C"
+
+

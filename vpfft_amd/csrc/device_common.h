@@ -38,6 +38,14 @@ struct StepConst {
   int nx, ny, nz, nzh;
 };
 
+// Read-only view of per-step constants: loads through the constant address
+// space are invariant for the kernel, so uniform ones become scalar loads
+// the compiler may repeat instead of holding the values in registers.
+template <typename T>
+__device__ __forceinline__ const T* as_const_mem(const T* p) {
+  return (const T*)(const __attribute__((address_space(4))) T*)p;
+}
+
 // FCC {111}<110> systems in the reference's order (BurgerVectorBasis.h:16-72).
 __device__ __constant__ static const signed char kB[kSlips][3] = {
     {0, 1, 1}, {1, 0, 1}, {1, -1, 0}, {0, 1, -1}, {1, 0, 1}, {1, 1, 0},

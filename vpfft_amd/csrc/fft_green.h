@@ -19,6 +19,7 @@ struct AxisPlan {
   int n;
   int nstages;
   int radix[12];
+  unsigned long long code;  // radix[s] in bits [4s, 4s+4): read with shifts, no indexed array
 };
 
 // Per-step Green-operator constants in the transform precision.
@@ -33,6 +34,48 @@ struct GreenConst {
 
 // Factor n into radices {8,4,2,3,5,7}; returns false for other prime factors.
 bool make_axis_plan(int n, AxisPlan* ap);
+// The same with the radices in the given order of preference (a permutation
+// of {8,4,2,3,5,7}); the stage order changes the rounding, not the result.
+bool make_axis_plan_ordered(int n, const int order[6], AxisPlan* ap);
+
+// One transform problem of a batched launch (a whole grid).
+struct FftJob {
+  FftGeom g;
+  AxisPlan ax, ay, az;
+  const void* tw[3];  // twiddles for x, y, z in the transform precision
+  const void* field;  // real input of the forward z pass (phi)
+  void* dtil;         // real output of the inverse z pass
+  void* spec;         // half spectrum
+  const void* gc;     // GreenConst<T>
+  const int* done;    // the job's loop flag (may be null)
+};
+// A batched launch: blocks [off[m], off[m+1]) belong to jobs[ids[m]].
+struct FftGroup {
+  const FftJob* jobs;
+  const int* ids;
+  const long* off;
+  int count;
+};
+
+// Block shape of a pass for one grid: B pencils (rows or runs) per block,
+// E values per thread (x pass), blocks per launch, dynamic LDS bytes.
+// B = 0 picks the default; other B values are checked against the LDS budget.
+struct PassShape {
+  int B, E;
+  long blocks;
+  size_t lds;
+  bool ok;
+};
+enum FftPass { kPassZ = 0, kPassY = 1, kPassX = 2 };
+PassShape pass_shape(int prec, FftPass pass, const FftGeom& g, int B);
+// candidate B values of a pass (for the tuner), returns the count
+int pass_candidates(FftPass pass, int* out, int cap);
+
+hipError_t launch_z_r2c_group(int prec, int B, const FftGroup& grp, long blocks, size_t lds, hipStream_t st);
+hipError_t launch_z_c2r_group(int prec, int B, const FftGroup& grp, long blocks, size_t lds, hipStream_t st);
+hipError_t launch_y_group(int prec, int B, const FftGroup& grp, long blocks, size_t lds, int sign, hipStream_t st);
+hipError_t launch_x_green_group(int prec, int B, int E, const FftGroup& grp, long blocks, size_t lds,
+                                hipStream_t st);
 
 // prec: 0 = FP64 storage/compute, 1 = FP32.  done: optional device flag; a
 // kernel whose flag is set returns at entry (graph-captured iterations after

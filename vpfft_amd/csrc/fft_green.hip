@@ -30,6 +30,12 @@
 #include "device_common.h"
 #include "fft_green.h"
 
+// No multiply-add contraction in this file: with it the compiler's choice of
+// which products to fuse can differ between the block-shape instantiations
+// of a pass, and a transform's bits would depend on its block shape.  The
+// passes are bound by HBM traffic, not by these instructions.
+#pragma STDC FP_CONTRACT OFF
+
 namespace vpfft {
 
 template <typename T> struct C2;
@@ -163,24 +169,41 @@ __device__ __forceinline__ void stockham_stage(const typename C2<T>::t* __restri
   }
 }
 
+// Values every lane of the block holds alike, moved to scalar registers so
+// the FFT's loop bounds and radix switches stay uniform control flow.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ long uni(long x) {
+  const unsigned long u = (unsigned long)x;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return (long)(((unsigned long)hi << 32) | lo);
+}
+template <typename P> __device__ __forceinline__ P* uni(P* p) { return (P*)uni((long)p); }
+// a uniform pointer read from a job table, known to address global memory
+template <typename P> __device__ __forceinline__ P* guni(P* p) {
+  return (P*)(__attribute__((address_space(1))) P*)uni(p);
+}
+
 // Full transform of P pencils of length n.  Data starts in buf0; returns the
 // buffer holding the result (buf0 or buf1).  Ends with a barrier.
 template <typename T, int P, int LS = P>
-__device__ typename C2<T>::t* lds_fft(typename C2<T>::t* buf0, typename C2<T>::t* buf1,
+__device__ __forceinline__ typename C2<T>::t* lds_fft(typename C2<T>::t* buf0, typename C2<T>::t* buf1,
                                       const AxisPlan& ap, const typename C2<T>::t* tw, int sign) {
   using C = typename C2<T>::t;
   C* in = buf0;
   C* out = buf1;
   int ns = 1;
-  for (int s = 0; s < ap.nstages; s++) {
-    const int r = ap.radix[s];
+  const int n = uni(ap.n), nstages = uni(ap.nstages);
+  const unsigned long long code = (unsigned long long)uni((long)ap.code);
+  for (int s = 0; s < nstages; s++) {
+    const int r = (int)((code >> (4 * s)) & 15);
     switch (r) {
-      case 2: stockham_stage<T, P, 2, LS>(in, out, ap.n, ns, tw, sign); break;
-      case 3: stockham_stage<T, P, 3, LS>(in, out, ap.n, ns, tw, sign); break;
-      case 4: stockham_stage<T, P, 4, LS>(in, out, ap.n, ns, tw, sign); break;
-      case 5: stockham_stage<T, P, 5, LS>(in, out, ap.n, ns, tw, sign); break;
-      case 7: stockham_stage<T, P, 7, LS>(in, out, ap.n, ns, tw, sign); break;
-      default: stockham_stage<T, P, 8, LS>(in, out, ap.n, ns, tw, sign); break;
+      case 2: stockham_stage<T, P, 2, LS>(in, out, n, ns, tw, sign); break;
+      case 3: stockham_stage<T, P, 3, LS>(in, out, n, ns, tw, sign); break;
+      case 4: stockham_stage<T, P, 4, LS>(in, out, n, ns, tw, sign); break;
+      case 5: stockham_stage<T, P, 5, LS>(in, out, n, ns, tw, sign); break;
+      case 7: stockham_stage<T, P, 7, LS>(in, out, n, ns, tw, sign); break;
+      default: stockham_stage<T, P, 8, LS>(in, out, n, ns, tw, sign); break;
     }
     __syncthreads();
     ns *= r;
@@ -196,17 +219,16 @@ __device__ typename C2<T>::t* lds_fft(typename C2<T>::t* buf0, typename C2<T>::t
 // Pencil p = pair * B + b holds f[2 pair] + i f[2 pair + 1] of row b.
 // ---------------------------------------------------------------------------
 template <typename T, int B>
-__global__ void __launch_bounds__(256) k_z_r2c(const T* __restrict__ f, typename C2<T>::t* __restrict__ s,
-                                               FftGeom g, AxisPlan ap, const typename C2<T>::t* __restrict__ tw,
-                                               const int* __restrict__ done) {
+__device__ __forceinline__ void z_r2c_body(const T* __restrict__ f, typename C2<T>::t* __restrict__ s,
+                                           const FftGeom& g, const AxisPlan& ap,
+                                           const typename C2<T>::t* __restrict__ tw, long blk) {
   using C = typename C2<T>::t;
-  if (done && *done) return;
   constexpr int P = 3 * B;
   constexpr int LS = P;
   extern __shared__ unsigned char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
   C* buf1 = buf0 + LS * g.nz;
-  const long row0 = (long)blockIdx.x * B;
+  const long row0 = blk * B;
   const int nrows = (int)min((long)B, g.nrows - row0);
   const int nz = g.nz, nzh = g.nzh, nzp = g.nzp;
   const int cnt = B * nz;
@@ -243,21 +265,28 @@ __global__ void __launch_bounds__(256) k_z_r2c(const T* __restrict__ f, typename
   }
 }
 
+template <typename T, int B>
+__global__ void __launch_bounds__(256) k_z_r2c(const T* __restrict__ f, typename C2<T>::t* __restrict__ s,
+                                               FftGeom g, AxisPlan ap, const typename C2<T>::t* __restrict__ tw,
+                                               const int* __restrict__ done) {
+  if (done && *done) return;
+  z_r2c_body<T, B>(f, s, g, ap, tw, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // z pass, half spectrum -> real, scaled by 1/N.
 // ---------------------------------------------------------------------------
 template <typename T, int B>
-__global__ void __launch_bounds__(256) k_z_c2r(const typename C2<T>::t* __restrict__ s, T* __restrict__ f,
-                                               FftGeom g, AxisPlan ap, const typename C2<T>::t* __restrict__ tw,
-                                               T scale, const int* __restrict__ done) {
+__device__ __forceinline__ void z_c2r_body(const typename C2<T>::t* __restrict__ s, T* __restrict__ f,
+                                           const FftGeom& g, const AxisPlan& ap,
+                                           const typename C2<T>::t* __restrict__ tw, T scale, long blk) {
   using C = typename C2<T>::t;
-  if (done && *done) return;
   constexpr int P = 3 * B;
   constexpr int LS = P;
   extern __shared__ unsigned char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
   C* buf1 = buf0 + LS * g.nz;
-  const long row0 = (long)blockIdx.x * B;
+  const long row0 = blk * B;
   const int nrows = (int)min((long)B, g.nrows - row0);
   const int nz = g.nz, nzh = g.nzh, nzp = g.nzp;
   const int cnth = B * nzh;
@@ -295,21 +324,27 @@ __global__ void __launch_bounds__(256) k_z_c2r(const typename C2<T>::t* __restri
   }
 }
 
+template <typename T, int B>
+__global__ void __launch_bounds__(256) k_z_c2r(const typename C2<T>::t* __restrict__ s, T* __restrict__ f,
+                                               FftGeom g, AxisPlan ap, const typename C2<T>::t* __restrict__ tw,
+                                               T scale, const int* __restrict__ done) {
+  if (done && *done) return;
+  z_c2r_body<T, B>(s, f, g, ap, tw, scale, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // y pass, in place.  Block = (component c, x, run of B consecutive kz).
 // ---------------------------------------------------------------------------
 template <typename T, int B>
-__global__ void __launch_bounds__(256) k_y(typename C2<T>::t* __restrict__ s, FftGeom g, AxisPlan ap,
-                                           const typename C2<T>::t* __restrict__ tw, int sign,
-                                           const int* __restrict__ done) {
+__device__ __forceinline__ void y_body(typename C2<T>::t* __restrict__ s, const FftGeom& g, const AxisPlan& ap,
+                                       const typename C2<T>::t* __restrict__ tw, int sign, long blk) {
   using C = typename C2<T>::t;
-  if (done && *done) return;
   extern __shared__ unsigned char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
   C* buf1 = buf0 + B * g.ny;
   const int runs = (g.nzh + B - 1) / B;
-  const int run = blockIdx.x % runs;
-  const long plane = blockIdx.x / runs;  // c * nx + x
+  const int run = (int)(blk % runs);
+  const long plane = blk / runs;  // c * nx + x
   const int kz0 = run * B;
   const int nb = min(B, g.nzh - kz0);
   C* base = s + plane * (long)g.ny * g.nzp + kz0;
@@ -324,6 +359,14 @@ __global__ void __launch_bounds__(256) k_y(typename C2<T>::t* __restrict__ s, Ff
     const int b = e % B, y = e / B;
     if (b < nb) base[(long)y * g.nzp + b] = res[e];
   }
+}
+
+template <typename T, int B>
+__global__ void __launch_bounds__(256) k_y(typename C2<T>::t* __restrict__ s, FftGeom g, AxisPlan ap,
+                                           const typename C2<T>::t* __restrict__ tw, int sign,
+                                           const int* __restrict__ done) {
+  if (done && *done) return;
+  y_body<T, B>(s, g, ap, tw, sign, blockIdx.x);
 }
 
 // Green operator applied to one frequency: phi <- G_eff(k) phi  (complex 5-vector).
@@ -429,18 +472,17 @@ __device__ __forceinline__ void green_apply(typename C2<T>::t ph[5], int ix, int
 // frequencies kept in registers (E per thread), Green applied, inverse FFT.
 // ---------------------------------------------------------------------------
 template <typename T, int B, int E>
-__global__ void __launch_bounds__(512) k_x_green(typename C2<T>::t* __restrict__ s, FftGeom g, AxisPlan ap,
-                                                 const typename C2<T>::t* __restrict__ tw,
-                                                 const GreenConst<T>* __restrict__ gc, const int* __restrict__ done) {
+__device__ __forceinline__ void x_green_body(typename C2<T>::t* __restrict__ s, const FftGeom& g, const AxisPlan& ap,
+                                             const typename C2<T>::t* __restrict__ tw,
+                                             const GreenConst<T>* __restrict__ gc, long blk) {
   using C = typename C2<T>::t;
-  if (done && *done) return;
   extern __shared__ unsigned char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
   C* buf1 = buf0 + B * g.nx;
   const long ncol = (long)g.ny * g.nzp;  // x stride
   const int runs = (g.nzh + B - 1) / B;
-  const int iy = blockIdx.x / runs;
-  const int kz0 = (blockIdx.x - iy * runs) * B;
+  const int iy = (int)(blk / runs);
+  const int kz0 = (int)(blk - (long)iy * runs) * B;
   const long q0 = (long)iy * g.nzp + kz0;
   const int nb = min(B, g.nzh - kz0);
   const int cnt = B * g.nx;
@@ -462,7 +504,7 @@ __global__ void __launch_bounds__(512) k_x_green(typename C2<T>::t* __restrict__
     }
     __syncthreads();
   }
-  const GreenConst<T> G = *gc;
+  const GreenConst<T> G = *as_const_mem(gc);
 #pragma unroll
   for (int i = 0; i < E; i++) {
     const int e = threadIdx.x + i * blockDim.x;
@@ -495,14 +537,101 @@ __global__ void __launch_bounds__(512) k_x_green(typename C2<T>::t* __restrict__
   }
 }
 
+template <typename T, int B, int E>
+__global__ void __launch_bounds__(512) k_x_green(typename C2<T>::t* __restrict__ s, FftGeom g, AxisPlan ap,
+                                                 const typename C2<T>::t* __restrict__ tw,
+                                                 const GreenConst<T>* __restrict__ gc, const int* __restrict__ done) {
+  if (done && *done) return;
+  x_green_body<T, B, E>(s, g, ap, tw, gc, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batched passes: one launch covers the same pass of many transform jobs
+// (independent grids of any size).  Blocks [off[m], off[m+1]) of the launch
+// belong to job ids[m]; a job whose loop flag is set returns at entry.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int group_member(const FftGroup& grp, long b, long* local) {
+  int lo = 0, hi = grp.count;  // off[lo] <= b < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (uni(grp.off[mid]) <= b) lo = mid;
+    else hi = mid;
+  }
+  *local = b - uni(grp.off[lo]);
+  return uni(grp.ids[lo]);
+}
+
+__device__ __forceinline__ FftGeom uni(const FftGeom& g) {
+  FftGeom r;
+  r.n = uni(g.n);
+  r.nc = uni(g.nc);
+  r.nrows = uni(g.nrows);
+  r.nx = uni(g.nx);
+  r.ny = uni(g.ny);
+  r.nz = uni(g.nz);
+  r.nzh = uni(g.nzh);
+  r.nzp = uni(g.nzp);
+  return r;
+}
+
+template <typename T, int B>
+__global__ void __launch_bounds__(256) k_z_r2c_b(FftGroup grp) {
+  using C = typename C2<T>::t;
+  long local;
+  const FftJob* J = grp.jobs + group_member(grp, blockIdx.x, &local);
+  const int* done = uni(J->done);
+  if (done && *done) return;
+  const FftGeom g = uni(J->g);
+  const AxisPlan& ap = J->az;  // read per stage, see lds_fft
+  z_r2c_body<T, B>(guni((const T*)J->field), guni((C*)J->spec), g, ap, guni((const C*)J->tw[2]), local);
+}
+template <typename T, int B>
+__global__ void __launch_bounds__(256) k_z_c2r_b(FftGroup grp) {
+  using C = typename C2<T>::t;
+  long local;
+  const FftJob* J = grp.jobs + group_member(grp, blockIdx.x, &local);
+  const int* done = uni(J->done);
+  if (done && *done) return;
+  const FftGeom g = uni(J->g);
+  const AxisPlan& ap = J->az;  // read per stage, see lds_fft
+  z_c2r_body<T, B>(guni((const C*)J->spec), guni((T*)J->dtil), g, ap, guni((const C*)J->tw[2]),
+                   (T)(1.0 / (double)g.n), local);
+}
+template <typename T, int B>
+__global__ void __launch_bounds__(256) k_y_b(FftGroup grp, int sign) {
+  using C = typename C2<T>::t;
+  long local;
+  const FftJob* J = grp.jobs + group_member(grp, blockIdx.x, &local);
+  const int* done = uni(J->done);
+  if (done && *done) return;
+  const FftGeom g = uni(J->g);
+  const AxisPlan& ap = J->ay;  // read per stage, see lds_fft
+  y_body<T, B>(guni((C*)J->spec), g, ap, guni((const C*)J->tw[1]), sign, local);
+}
+template <typename T, int B, int E>
+__global__ void __launch_bounds__(512) k_x_green_b(FftGroup grp) {
+  using C = typename C2<T>::t;
+  long local;
+  const FftJob* J = grp.jobs + group_member(grp, blockIdx.x, &local);
+  const int* done = uni(J->done);
+  if (done && *done) return;
+  const FftGeom g = uni(J->g);
+  const AxisPlan& ap = J->ax;  // read per stage, see lds_fft
+  x_green_body<T, B, E>(guni((C*)J->spec), g, ap, guni((const C*)J->tw[0]), uni((const GreenConst<T>*)J->gc), local);
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
 bool make_axis_plan(int n, AxisPlan* ap) {
+  const int order[6] = {8, 4, 2, 3, 5, 7};
+  return make_axis_plan_ordered(n, order, ap);
+}
+
+bool make_axis_plan_ordered(int n, const int order[6], AxisPlan* ap) {
   ap->n = n;
   ap->nstages = 0;
   int m = n;
-  const int order[6] = {8, 4, 2, 3, 5, 7};
   for (int oi = 0; oi < 6; oi++) {
     const int r = order[oi];
     while (m % r == 0 && m > 1) {
@@ -511,6 +640,8 @@ bool make_axis_plan(int n, AxisPlan* ap) {
       m /= r;
     }
   }
+  ap->code = 0;
+  for (int s = 0; s < ap->nstages; s++) ap->code |= (unsigned long long)ap->radix[s] << (4 * s);
   return m == 1 && n > 1;
 }
 
@@ -620,6 +751,114 @@ hipError_t launch_x_green(int prec, void* s, const FftGeom& g, const AxisPlan& a
                           const int* done, hipStream_t st) {
   return prec == 0 ? launch_x_green_t<double>(s, g, ap, tw, gc, done, st)
                    : launch_x_green_t<float>(s, g, ap, tw, gc, done, st);
+}
+
+// ---- batched launches ------------------------------------------------------
+static constexpr size_t kMaxLds = 65536;
+
+int pass_candidates(FftPass pass, int* out, int cap) {
+  static const int z[] = {1, 2, 4, 8, 16};
+  static const int y[] = {1, 2, 4, 8, 16, 32, 64};
+  static const int x[] = {8, 16, 32, 64};
+  const int* src = pass == kPassZ ? z : (pass == kPassY ? y : x);
+  const int cnt = pass == kPassZ ? 5 : (pass == kPassY ? 7 : 4);
+  int k = 0;
+  for (; k < cnt && k < cap; k++) out[k] = src[k];
+  return k;
+}
+
+PassShape pass_shape(int prec, FftPass pass, const FftGeom& g, int B) {
+  const size_t csz = prec == 0 ? 16 : 8;
+  PassShape ps{0, 1, 0, 0, false};
+  if (pass == kPassZ) {
+    ps.B = B > 0 ? B : pick_rows(g.nz, prec == 0 ? 1536 : 3072);
+    ps.blocks = (g.nrows + ps.B - 1) / ps.B;
+    ps.lds = 2 * csz * 3 * ps.B * g.nz;
+  } else if (pass == kPassY) {
+    ps.B = B > 0 ? B : pick_run(g.ny, prec == 0 ? 2048 : 4096);
+    ps.blocks = 5L * g.nx * ((g.nzh + ps.B - 1) / ps.B);
+    ps.lds = 2 * csz * ps.B * g.ny;
+  } else {
+    int b = B;
+    if (b <= 0) {
+      const int minrun = prec == 0 ? 8 : 16;
+      b = pick_run(g.nx, 2048);
+      if (b < minrun) b = minrun;
+    }
+    ps.B = b;
+    ps.blocks = (long)g.ny * ((g.nzh + b - 1) / b);
+    ps.lds = 2 * csz * b * g.nx;
+    ps.E = (b * g.nx + 511) / 512;
+    if (ps.E > 8) return ps;  // register budget of the x pass
+    ps.E = ps.E <= 1 ? 1 : (ps.E <= 2 ? 2 : (ps.E <= 4 ? 4 : 8));
+  }
+  ps.ok = ps.lds <= kMaxLds;
+  return ps;
+}
+
+template <typename T>
+static hipError_t z_r2c_group_t(int B, const FftGroup& grp, long blocks, size_t lds, hipStream_t st) {
+  switch (B) {
+#define CASE(BB) case BB: hipLaunchKernelGGL((k_z_r2c_b<T, BB>), dim3((unsigned)blocks), dim3(256), lds, st, grp); break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+template <typename T>
+static hipError_t z_c2r_group_t(int B, const FftGroup& grp, long blocks, size_t lds, hipStream_t st) {
+  switch (B) {
+#define CASE(BB) case BB: hipLaunchKernelGGL((k_z_c2r_b<T, BB>), dim3((unsigned)blocks), dim3(256), lds, st, grp); break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+template <typename T>
+static hipError_t y_group_t(int B, const FftGroup& grp, long blocks, size_t lds, int sign, hipStream_t st) {
+  switch (B) {
+#define CASE(BB) case BB: hipLaunchKernelGGL((k_y_b<T, BB>), dim3((unsigned)blocks), dim3(256), lds, st, grp, sign); break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
+#undef CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+template <typename T>
+static hipError_t x_group_t(int B, int E, const FftGroup& grp, long blocks, size_t lds, hipStream_t st) {
+#define LAUNCH(BB, EE) hipLaunchKernelGGL((k_x_green_b<T, BB, EE>), dim3((unsigned)blocks), dim3(512), lds, st, grp)
+#define EC(BB)                                  \
+  case BB:                                      \
+    if (E == 1) LAUNCH(BB, 1);                  \
+    else if (E == 2) LAUNCH(BB, 2);             \
+    else if (E == 4) LAUNCH(BB, 4);             \
+    else if (E == 8) LAUNCH(BB, 8);             \
+    else return hipErrorInvalidValue;           \
+    break;
+  switch (B) {
+    EC(8) EC(16) EC(32) EC(64)
+    default: return hipErrorInvalidValue;
+  }
+#undef EC
+#undef LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_z_r2c_group(int prec, int B, const FftGroup& grp, long blocks, size_t lds, hipStream_t st) {
+  return prec == 0 ? z_r2c_group_t<double>(B, grp, blocks, lds, st) : z_r2c_group_t<float>(B, grp, blocks, lds, st);
+}
+hipError_t launch_z_c2r_group(int prec, int B, const FftGroup& grp, long blocks, size_t lds, hipStream_t st) {
+  return prec == 0 ? z_c2r_group_t<double>(B, grp, blocks, lds, st) : z_c2r_group_t<float>(B, grp, blocks, lds, st);
+}
+hipError_t launch_y_group(int prec, int B, const FftGroup& grp, long blocks, size_t lds, int sign, hipStream_t st) {
+  return prec == 0 ? y_group_t<double>(B, grp, blocks, lds, sign, st)
+                   : y_group_t<float>(B, grp, blocks, lds, sign, st);
+}
+hipError_t launch_x_green_group(int prec, int B, int E, const FftGroup& grp, long blocks, size_t lds,
+                                hipStream_t st) {
+  return prec == 0 ? x_group_t<double>(B, E, grp, blocks, lds, st) : x_group_t<float>(B, E, grp, blocks, lds, st);
 }
 
 }  // namespace vpfft

@@ -14,7 +14,7 @@ constexpr int kNrWaves = 2;        // waves per SIMD the Newton kernel is compil
 constexpr int kPartialStride = 24;  // 15 sums + 2 maxima per block, padded
 
 // the Newton kernel runs one voxel per lane
-inline int nr_blocks(long n) { return (int)((n + kPointThreads - 1) / kPointThreads); }
+__host__ __device__ inline int nr_blocks(long n) { return (int)((n + kPointThreads - 1) / kPointThreads); }
 // partial-sum buffer rows needed by launch_nr (+ its folded rows)
 inline long nr_partial_rows(long n) { return nr_blocks(n) + (nr_blocks(n) + 255) / 256; }
 
@@ -79,5 +79,41 @@ hipError_t launch_constrained_batch(long n, const double* R, const double* crss,
                                     double* err, int* steps, hipStream_t st);
 hipError_t launch_taylor_batch(long n, const double* R, const double* crss, const StepConst* C, const double* s_in,
                                double* s_out, hipStream_t st);
+
+// One grid of a batched pointwise launch.  dtil / phi are in the group's
+// transform precision; partial holds nr_blocks(n) rows of kPartialStride
+// (and of 18 for the prologue).
+struct VoxJob {
+  long n;
+  const void* dtil;
+  void* phi;
+  double* sigma;
+  double* lambda;
+  double* R;
+  double* crss;
+  double* acc;
+  StepConst* C;
+  GreenConst<double>* g64;
+  GreenConst<float>* g32;
+  double* partial;
+  LoopState* ls;
+  IterRecord* rec;
+  PrologueRecord* prol;
+};
+// A batched launch: blocks [off[m], off[m+1]) belong to jobs[ids[m]]; the
+// one-block-per-job kernels use ids[0..count).
+struct VoxGroup {
+  const VoxJob* jobs;
+  const int* ids;
+  const long* off;
+  int count;
+};
+hipError_t launch_prologue_group(int prec, const VoxGroup& grp, long blocks, hipStream_t st);
+hipError_t launch_L0_finalize_group(const VoxGroup& grp, hipStream_t st);
+hipError_t launch_loop_reset_group(const VoxGroup& grp, hipStream_t st);
+hipError_t launch_nr_group(int prec, int nexp_uniform, bool uniform_tau, const VoxGroup& grp, long blocks,
+                           hipStream_t st);
+hipError_t launch_iter_finalize_group(const VoxGroup& grp, double tol, int max_iter, hipStream_t st);
+hipError_t launch_epilogue_group(const VoxGroup& grp, long blocks, hipStream_t st);
 
 }  // namespace vpfft

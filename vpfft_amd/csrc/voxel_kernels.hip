@@ -84,18 +84,14 @@ __device__ __forceinline__ void spd_inverse5(const double A[15], double Ainv[15]
 }
 
 // ---------------------------------------------------------------------------
+// InitializeComputation + the secant moduli of BuildLinearRefMediumStiffness
+// for voxel v; adds M_loc^-1 (15) and the degenerate count (slot 15) to acc.
 template <typename T>
-__global__ void __launch_bounds__(kPointThreads) k_prologue(long n, const double* __restrict__ R,
-                                                            const double* __restrict__ crss,
-                                                            const StepConst* __restrict__ Cp, T* __restrict__ field,
-                                                            double* __restrict__ sigma, double* __restrict__ lambda,
-                                                            double* __restrict__ partial) {
-  const StepConst& C = *Cp;
-  double acc[17];
-#pragma unroll
-  for (int w = 0; w < 17; w++) acc[w] = 0.0;
-  double mx[1] = {0.0};
-  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void prologue_voxel(long v, long n, const double* __restrict__ R,
+                                               const double* __restrict__ crss, const StepConst& C,
+                                               T* __restrict__ field, double* __restrict__ sigma,
+                                               double* __restrict__ lambda, double (&acc)[17]) {
+  {
     double r[9], tau[kSlips], it[kSlips], m[kSlips][5];
     load_R(R, n, v, r);
 #pragma unroll
@@ -140,6 +136,21 @@ __global__ void __launch_bounds__(kPointThreads) k_prologue(long n, const double
 #pragma unroll
     for (int e = 0; e < 15; e++) acc[e] += Mi[e];
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kPointThreads) k_prologue(long n, const double* __restrict__ R,
+                                                            const double* __restrict__ crss,
+                                                            const StepConst* __restrict__ Cp, T* __restrict__ field,
+                                                            double* __restrict__ sigma, double* __restrict__ lambda,
+                                                            double* __restrict__ partial) {
+  const StepConst& C = *Cp;
+  double acc[17];
+#pragma unroll
+  for (int w = 0; w < 17; w++) acc[w] = 0.0;
+  double mx[1] = {0.0};
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x)
+    prologue_voxel<T>(v, n, R, crss, C, field, sigma, lambda, acc);
   block_sum<17>(acc, mx, partial + (long)blockIdx.x * 18);
 }
 
@@ -191,9 +202,10 @@ __device__ void derive_constants(StepConst* C, GreenConst<double>* g64, GreenCon
   g64->nz = g32->nz = C->nz;
 }
 
-__global__ void __launch_bounds__(256) k_L0_finalize(long n, int nblocks, const double* __restrict__ partial,
-                                                     StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32,
-                                                     PrologueRecord* rec) {
+// L0 from nblocks prologue partial rows (18 wide); one block.
+__device__ __forceinline__ void l0_finalize_body(long n, int nblocks, const double* __restrict__ partial,
+                                                 StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32,
+                                                 PrologueRecord* rec) {
   __shared__ double sh[256][18];
   double acc[18];
   for (int w = 0; w < 18; w++) acc[w] = 0.0;
@@ -222,6 +234,13 @@ __global__ void __launch_bounds__(256) k_L0_finalize(long n, int nblocks, const 
   }
 }
 
+
+__global__ void __launch_bounds__(256) k_L0_finalize(long n, int nblocks, const double* __restrict__ partial,
+                                                     StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32,
+                                                     PrologueRecord* rec) {
+  l0_finalize_body(n, nblocks, partial, C, g64, g32, rec);
+}
+
 __global__ void k_loop_reset(LoopState* ls) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     ls->iter = 0;
@@ -238,6 +257,53 @@ __global__ void k_derive(StepConst* C, GreenConst<double>* g64, GreenConst<float
   if (threadIdx.x == 0 && blockIdx.x == 0) derive_constants(C, g64, g32);
 }
 
+// values alike in every lane of the block, moved to scalar registers
+__device__ __forceinline__ int vuni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ long vuni(long x) {
+  const unsigned long u = (unsigned long)x;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return (long)(((unsigned long)hi << 32) | lo);
+}
+template <typename P> __device__ __forceinline__ P* vuni(P* p) { return (P*)vuni((long)p); }
+
+__device__ __forceinline__ int vox_member(const VoxGroup& grp, long b, long* local) {
+  int lo = 0, hi = grp.count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (vuni(grp.off[mid]) <= b) lo = mid;
+    else hi = mid;
+  }
+  *local = b - vuni(grp.off[lo]);
+  return vuni(grp.ids[lo]);
+}
+
+// generic pointers from a table, known to address global memory
+template <typename P> __device__ __forceinline__ P* as_global(P* p) {
+  return (P*)(__attribute__((address_space(1))) P*)p;
+}
+
+// the job's fields in scalar registers
+__device__ __forceinline__ VoxJob vox_job(const VoxJob* J) {
+  VoxJob r;
+  r.n = vuni(J->n);
+  r.dtil = vuni(J->dtil);
+  r.phi = vuni(J->phi);
+  r.sigma = as_global(vuni(J->sigma));
+  r.lambda = as_global(vuni(J->lambda));
+  r.R = as_global(vuni(J->R));
+  r.crss = as_global(vuni(J->crss));
+  r.acc = as_global(vuni(J->acc));
+  r.C = vuni(J->C);
+  r.g64 = vuni(J->g64);
+  r.g32 = vuni(J->g32);
+  r.partial = as_global(vuni(J->partial));
+  r.ls = vuni(J->ls);
+  r.rec = vuni(J->rec);
+  r.prol = vuni(J->prol);
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // K4: per-voxel body of the fixed-point loop (MaterialGrid.cpp:946-1003),
 // with the Newton solve in the crystal frame of the voxel.
@@ -246,19 +312,51 @@ __global__ void k_derive(StepConst* C, GreenConst<double>* g64, GreenConst<float
 // per-wave max NR steps; then max NR error, max NR steps
 // NE > 0: every slip system has rate-sensitivity exponent NE.
 // ---------------------------------------------------------------------------
-template <typename T, int NE, bool UNIFORM_TAU>
+// BATCHED: the grid's pointers come from the job table of grp (blocks of
+// many grids in one launch); else from the kernel arguments.
+template <typename T, int NE, bool UNIFORM_TAU, bool BATCHED>
 __global__ void __launch_bounds__(kPointThreads) __attribute__((amdgpu_waves_per_eu(kNrWaves, kNrWaves)))
-k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi, double* __restrict__ sigma,
-     double* __restrict__ lambda, const double* __restrict__ R, const double* __restrict__ crss,
-     const StepConst* __restrict__ Cp, double* __restrict__ partial, const int* __restrict__ done) {
-  if (done && *done) return;
-  const StepConst& C = *Cp;
+k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restrict__ sigma_,
+     double* __restrict__ lambda_, const double* __restrict__ R_, const double* __restrict__ crss_,
+     const StepConst* __restrict__ Cp_, double* __restrict__ partial_, const int* __restrict__ done, VoxGroup grp) {
+  long n, blk;
+  const T* dtil;
+  T* phi;
+  double *sigma, *lambda, *partial;
+  const double *R, *crss;
+  const StepConst* Cp;
+  if constexpr (BATCHED) {
+    const VoxJob J = vox_job(grp.jobs + vox_member(grp, blockIdx.x, &blk));
+    if (J.ls->done) return;
+    n = J.n;
+    dtil = as_global((const T*)J.dtil);
+    phi = as_global((T*)J.phi);
+    sigma = as_global(J.sigma);
+    lambda = as_global(J.lambda);
+    R = as_global((const double*)J.R);
+    crss = as_global((const double*)J.crss);
+    Cp = J.C;
+    partial = as_global(J.partial);
+  } else {
+    if (done && *done) return;
+    n = n_;
+    dtil = dtil_;
+    phi = phi_;
+    sigma = sigma_;
+    lambda = lambda_;
+    R = R_;
+    crss = crss_;
+    Cp = Cp_;
+    partial = partial_;
+    blk = blockIdx.x;
+  }
+  const StepConst& C = *as_const_mem(Cp);
   double acc[15];
 #pragma unroll
   for (int w = 0; w < 15; w++) acc[w] = 0.0;
   double mx[2] = {0.0, 0.0};  // max NR error, max NR steps
   // one voxel per lane: nothing but the voxel's own state is live
-  const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long v = blk * blockDim.x + threadIdx.x;
   if (v < n) {
     double it[kSlips], r9[9];
     load_R(R, n, v, r9);
@@ -311,6 +409,8 @@ k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi, double* __restrict
       cf::apply_constitutive<NE>(sc, it, C, ec);
       cf::rot_t(Q, ec, e);
     }
+    // re-read d~ and lambda here rather than keep them live through the solve
+    __asm__ volatile("" ::: "memory");
     double d[5], lam[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) {
@@ -348,8 +448,9 @@ k_nr(long n, const T* __restrict__ dtil, T* __restrict__ phi, double* __restrict
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
   if ((threadIdx.x & 63) == 0) acc[14] = 64.0 * wmax;
-  block_sum<15>(acc, mx, partial + (long)blockIdx.x * kPartialStride);
+  block_sum<15>(acc, mx, partial + blk * kPartialStride);
 }
+
 
 // Reduce rows [256 b, 256 b + 256) of the per-block partials into row b of
 // out: 15 sums then 2 maxima, fixed tree order.
@@ -373,9 +474,10 @@ __global__ void __launch_bounds__(256) k_fold_partials(const double* __restrict_
 
 // LoopState: the iteration counter and the done flag, both on the device so
 // a captured graph of several iterations needs no host round trip.
-__global__ void __launch_bounds__(256) k_iter_finalize(long n, int nblocks, const double* __restrict__ partial,
-                                                       const StepConst* __restrict__ C, double tol, int max_iter,
-                                                       LoopState* ls, IterRecord* rec) {
+// The fixed-point convergence test from nblocks rows of NR partials (one block).
+__device__ __forceinline__ void iter_finalize_body(long n, int nblocks, const double* __restrict__ partial,
+                                                   const StepConst* __restrict__ C, double tol, int max_iter,
+                                                   LoopState* ls, IterRecord* rec) {
   if (ls->done) return;
   const int it = ls->iter;
   __shared__ double sh[256][17];
@@ -424,16 +526,20 @@ __global__ void __launch_bounds__(256) k_iter_finalize(long n, int nblocks, cons
   }
 }
 
+__global__ void __launch_bounds__(256) k_iter_finalize(long n, int nblocks, const double* __restrict__ partial,
+                                                       const StepConst* __restrict__ C, double tol, int max_iter,
+                                                       LoopState* ls, IterRecord* rec) {
+  iter_finalize_body(n, nblocks, partial, C, tol, max_iter, ls, rec);
+}
+
 // ---------------------------------------------------------------------------
 // Epilogue: R += dt * sum_k (R a0_k R^T) gdot_k sgn(r_k) with the old Schmid
 // tensors, then Voce hardening with the new ones.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kPointThreads) k_epilogue(long n, double* __restrict__ R, double* __restrict__ crss,
-                                                            double* __restrict__ acc_shear,
-                                                            const double* __restrict__ sigma,
-                                                            const StepConst* __restrict__ Cp) {
-  const StepConst& C = *Cp;
-  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void epilogue_voxel(long v, long n, double* __restrict__ R, double* __restrict__ crss,
+                                               double* __restrict__ acc_shear, const double* __restrict__ sigma,
+                                               const StepConst& C) {
+  {
     double r[9], tau[kSlips], it[kSlips], m[kSlips][5], s[5];
     load_R(R, n, v, r);
 #pragma unroll
@@ -490,6 +596,58 @@ __global__ void __launch_bounds__(kPointThreads) k_epilogue(long n, double* __re
     }
     acc_shear[v] = A + local;
   }
+}
+
+__global__ void __launch_bounds__(kPointThreads) k_epilogue(long n, double* __restrict__ R, double* __restrict__ crss,
+                                                            double* __restrict__ acc_shear,
+                                                            const double* __restrict__ sigma,
+                                                            const StepConst* __restrict__ Cp) {
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x)
+    epilogue_voxel(v, n, R, crss, acc_shear, sigma, *Cp);
+}
+
+// ---------------------------------------------------------------------------
+// Batched pointwise kernels: one launch covers many grids (jobs).  Blocks
+// [off[m], off[m+1]) belong to jobs[ids[m]], one voxel per thread, so each
+// job's partial rows and their reduction order depend on that job alone.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kPointThreads) k_prologue_b(VoxGroup grp) {
+  long local;
+  const VoxJob J = vox_job(grp.jobs + vox_member(grp, blockIdx.x, &local));
+  double acc[17];
+#pragma unroll
+  for (int w = 0; w < 17; w++) acc[w] = 0.0;
+  double mx[1] = {0.0};
+  const long v = local * kPointThreads + threadIdx.x;
+  if (v < J.n) prologue_voxel<T>(v, J.n, J.R, J.crss, *as_const_mem(J.C), (T*)J.phi, J.sigma, J.lambda, acc);
+  block_sum<17>(acc, mx, J.partial + local * 18);
+}
+
+__global__ void __launch_bounds__(256) k_L0_finalize_b(VoxGroup grp) {
+  const VoxJob J = vox_job(grp.jobs + vuni(grp.ids[blockIdx.x]));
+  l0_finalize_body(J.n, nr_blocks(J.n), J.partial, J.C, J.g64, J.g32, J.prol);
+}
+
+__global__ void k_loop_reset_b(VoxGroup grp) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < grp.count) {
+    LoopState* ls = grp.jobs[grp.ids[m]].ls;
+    ls->iter = 0;
+    ls->done = 0;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_iter_finalize_b(VoxGroup grp, double tol, int max_iter) {
+  const VoxJob J = vox_job(grp.jobs + vuni(grp.ids[blockIdx.x]));
+  iter_finalize_body(J.n, nr_blocks(J.n), J.partial, J.C, tol, max_iter, J.ls, J.rec);
+}
+
+__global__ void __launch_bounds__(kPointThreads) k_epilogue_b(VoxGroup grp) {
+  long local;
+  const VoxJob J = vox_job(grp.jobs + vox_member(grp, blockIdx.x, &local));
+  const long v = local * kPointThreads + threadIdx.x;
+  if (v < J.n) epilogue_voxel(v, J.n, J.R, J.crss, J.acc, J.sigma, *as_const_mem(J.C));
 }
 
 // ---------------------------------------------------------------------------
@@ -590,8 +748,8 @@ hipError_t launch_nr(int prec, int nexp_uniform, bool uniform_tau, long n, const
                      const int* done, hipStream_t st) {
   const int nb = nr_blocks(n);
 #define NR_LAUNCH(TT, NN, UU)                                                                                    \
-  hipLaunchKernelGGL((k_nr<TT, NN, UU>), dim3(nb), dim3(kPointThreads), 0, st, n, (const TT*)dtil, (TT*)phi, sigma, \
-                     lambda, R, crss, C, partial, done)
+  hipLaunchKernelGGL((k_nr<TT, NN, UU, false>), dim3(nb), dim3(kPointThreads), 0, st, n, (const TT*)dtil, (TT*)phi, \
+                     sigma, lambda, R, crss, C, partial, done, VoxGroup{})
 #define NR_N(TT, UU)                                 \
   switch (nexp_uniform) {                             \
     case 5: NR_LAUNCH(TT, 5, UU); break;              \
@@ -672,6 +830,52 @@ hipError_t launch_constrained_batch(long n, const double* R, const double* crss,
 hipError_t launch_taylor_batch(long n, const double* R, const double* crss, const StepConst* C, const double* s_in,
                                double* s_out, hipStream_t st) {
   hipLaunchKernelGGL(k_taylor_batch, dim3(point_blocks(n)), dim3(kPointThreads), 0, st, n, R, crss, C, s_in, s_out);
+  return hipGetLastError();
+}
+
+// ---- batched launches --------------------------------------------------------
+hipError_t launch_prologue_group(int prec, const VoxGroup& grp, long blocks, hipStream_t st) {
+  if (prec == 0)
+    hipLaunchKernelGGL(k_prologue_b<double>, dim3((unsigned)blocks), dim3(kPointThreads), 0, st, grp);
+  else
+    hipLaunchKernelGGL(k_prologue_b<float>, dim3((unsigned)blocks), dim3(kPointThreads), 0, st, grp);
+  return hipGetLastError();
+}
+hipError_t launch_L0_finalize_group(const VoxGroup& grp, hipStream_t st) {
+  hipLaunchKernelGGL(k_L0_finalize_b, dim3(grp.count), dim3(256), 0, st, grp);
+  return hipGetLastError();
+}
+hipError_t launch_loop_reset_group(const VoxGroup& grp, hipStream_t st) {
+  hipLaunchKernelGGL(k_loop_reset_b, dim3((grp.count + 63) / 64), dim3(64), 0, st, grp);
+  return hipGetLastError();
+}
+hipError_t launch_nr_group(int prec, int nexp_uniform, bool uniform_tau, const VoxGroup& grp, long blocks,
+                           hipStream_t st) {
+#define NR_LAUNCH(TT, NN, UU)                                                                                   \
+  hipLaunchKernelGGL((k_nr<TT, NN, UU, true>), dim3((unsigned)blocks), dim3(kPointThreads), 0, st, 0L,          \
+                     (const TT*)nullptr, (TT*)nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, grp)
+#define NR_N(TT, UU)                                 \
+  switch (nexp_uniform) {                             \
+    case 5: NR_LAUNCH(TT, 5, UU); break;              \
+    case 10: NR_LAUNCH(TT, 10, UU); break;            \
+    case 20: NR_LAUNCH(TT, 20, UU); break;            \
+    default: NR_LAUNCH(TT, 0, UU); break;             \
+  }
+  if (prec == 0) {
+    if (uniform_tau) { NR_N(double, true) } else { NR_N(double, false) }
+  } else {
+    if (uniform_tau) { NR_N(float, true) } else { NR_N(float, false) }
+  }
+#undef NR_N
+#undef NR_LAUNCH
+  return hipGetLastError();
+}
+hipError_t launch_iter_finalize_group(const VoxGroup& grp, double tol, int max_iter, hipStream_t st) {
+  hipLaunchKernelGGL(k_iter_finalize_b, dim3(grp.count), dim3(256), 0, st, grp, tol, max_iter);
+  return hipGetLastError();
+}
+hipError_t launch_epilogue_group(const VoxGroup& grp, long blocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_epilogue_b, dim3((unsigned)blocks), dim3(kPointThreads), 0, st, grp);
   return hipGetLastError();
 }
 
