@@ -91,6 +91,9 @@ typedef struct {
   double nr_wave_steps_total;                  /* sum over iterations and 64-voxel waves of 64 x the wave's
                                                   largest step count: nr_steps_total / this = the Newton
                                                   loop's lane efficiency */
+  double err_ratio_5;                          /* max(err_stress, err_strain_rate) of the last iteration
+                                                  over the same five iterations earlier (1 with fewer):
+                                                  near 1 = the fixed point has stalled */
 } vpfft_step_info;
 
 /* ---- grid lifetime: MaterialGrid(int,int,int) + InitializeMemory (MaterialGrid.cpp:312-394) */
@@ -183,6 +186,12 @@ int vpfft_grid_bytes(const vpfft_grid_desc* desc, uint64_t* bytes_out);
  * allocated on first use (32 doubles per voxel). */
 int vpfft_grid_save_state(void* grid);
 int vpfft_grid_restore_state(void* grid);
+
+/* FFT block shape (pencils per block) of one pass (0 = z, 1 = y, 2 = x) for
+ * one precision of a grid; 0 restores the default (no reference
+ * counterpart).  Changes how pencils are grouped into blocks, not the
+ * arithmetic. */
+int vpfft_grid_set_block_shape(void* grid, int precision, int pass, int block);
 
 /* ---- batched strain steps (no reference counterpart: the reference runs one
  * grid per MPI job).  A batch advances several grids -- independent RVEs of

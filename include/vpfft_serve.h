@@ -6,8 +6,9 @@
  * slot count allow, keeps every admitted RVE's fields resident on the GPU
  * between its increments, and advances all admitted RVEs by one increment per
  * batched step, so requests that arrive while others run join the next step
- * (continuous batching).  A mixed-precision step that does not converge is
- * rerun in FP64 from the saved state.  A request's results do not depend on
+ * (continuous batching).  A mixed-precision step that ends unconverged with
+ * its residual stalled (no longer decreasing) is rerun in FP64 from the saved
+ * state.  A request's results do not depend on
  * which other requests share its steps. */
 #ifndef VPFFT_SERVE_H_
 #define VPFFT_SERVE_H_
@@ -38,7 +39,7 @@ typedef struct {
   double latency_s;    /* finished - arrival */
   int increments_done;
   int fp64_reruns;     /* increments rerun in FP64 */
-  int iterations;      /* fixed-point iterations over all increments */
+  int iterations;      /* fixed-point iterations over all increments, FP64 reruns included */
   int status;          /* VPFFT_OK or the error that ended it */
 } vpfft_request_result;
 
@@ -46,7 +47,9 @@ typedef struct {
   uint64_t hbm_budget_bytes;  /* device bytes the admitted grids may hold */
   int max_active;             /* admitted requests at once */
   int precision;              /* VPFFT_MIXED or VPFFT_FP64 */
-  int fp64_fallback;          /* 1: rerun a non-converged mixed step in FP64 */
+  int fp64_fallback;          /* 1: rerun a mixed step in FP64 when it ends unconverged and stalled */
+  double stall_ratio;         /* stalled: vpfft_step_info.err_ratio_5 >= stall_ratio (0 = always rerun
+                                 an unconverged mixed step) */
   int max_iter;
   double tol;
   int tune;                   /* FFT block-shape tuning of the batch */

@@ -55,8 +55,9 @@ def test_serve_admits_by_budget_and_slots(gpu, vpfft):
 def test_serve_fp64_rerun_of_unconverged_mixed_steps(gpu):
     from vpfft_amd import serve as S
     q = [_req(0.0, 16, 8, 3, 10, 2, 31)]
-    # a tolerance the loop cannot meet in 6 iterations: every mixed step is rerun in FP64
-    res, st, avg = S.serve(q, max_iter=6, tol=1e-14, fp64_fallback=True)
+    # a tolerance the loop cannot meet in 6 iterations, every unconverged step counted as stalled:
+    # every mixed step is rerun in FP64
+    res, st, avg = S.serve(q, max_iter=6, tol=1e-14, fp64_fallback=True, stall_ratio=0.0)
     assert res[0]["status"] == 0 and res[0]["fp64_reruns"] == 2 and st["fp64_reruns"] == 2
     res64, _, avg64 = S.serve(q, max_iter=6, tol=1e-14, precision=S.FP64, fp64_fallback=False)
     assert res64[0]["fp64_reruns"] == 0

@@ -85,6 +85,7 @@ class vpfft_step_info(C.Structure):
         ("ms_epilogue", C.c_double),
         ("max_nr_steps", C.c_double),
         ("nr_wave_steps_total", C.c_double),
+        ("err_ratio_5", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -123,7 +124,7 @@ EXPORTED = [
     "vpfft_solve_taylor_batch", "vpfft_generate_voronoi", "vpfft_last_error", "vpfft_version",
     "vpfft_grid_set_profiling", "vpfft_grid_kernel_times", "vpfft_kernel_name",
     "vpfft_batch_create", "vpfft_batch_destroy", "vpfft_batch_set_tuning", "vpfft_batch_block_shape",
-    "vpfft_batch_run_strain_step", "vpfft_grid_bytes", "vpfft_grid_save_state", "vpfft_grid_restore_state",
+    "vpfft_batch_run_strain_step", "vpfft_grid_set_block_shape", "vpfft_grid_bytes", "vpfft_grid_save_state", "vpfft_grid_restore_state",
     # include/vpfft_io.h
     "vpfft_deck_read", "vpfft_deck_parse", "vpfft_deck_material", "vpfft_deck_strain_rate",
     "vpfft_two_grain_init", "vpfft_euler_deg_to_matrix", "vpfft_read_euler_file", "vpfft_write_euler_file",
@@ -174,6 +175,7 @@ def load_library(path: Optional[str] = None):
                                                 dp, C.POINTER(vpfft_step_info)]
     lib.vpfft_grid_bytes.argtypes = [C.POINTER(vpfft_grid_desc), C.POINTER(C.c_uint64)]
     lib.vpfft_grid_save_state.argtypes = [vp]
+    lib.vpfft_grid_set_block_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     lib.vpfft_grid_restore_state.argtypes = [vp]
     lib.vpfft_deck_read.argtypes = [cp, C.POINTER(vpfft_deck)]
     lib.vpfft_deck_parse.argtypes = [cp, C.POINTER(vpfft_deck)]
@@ -363,6 +365,10 @@ class MaterialGrid:
         out = np.empty(3)
         _check(self._lib.vpfft_grid_get_voxel_length(self._h, _dptr(out)))
         return out
+
+    def SetBlockShape(self, precision: int, pass_: int, block: int):
+        """FFT pencils per block of a pass (0 z, 1 y, 2 x); 0 = default."""
+        _check(self._lib.vpfft_grid_set_block_shape(self._h, precision, pass_, block))
 
     def SetProfiling(self, enable: bool = True):
         """Per-kernel device timing with HIP events on the grid's stream (resets totals)."""

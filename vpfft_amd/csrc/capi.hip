@@ -98,6 +98,7 @@ struct Grid {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool uploaded = false;
   int nexp_uniform = 0;  // the exponent every slip shares (Newton kernel specialisation), else 0
+  int fftB[2][3] = {{0, 0, 0}, {0, 0, 0}};  // FFT block shape per precision and pass (z, y, x); 0 = default
   // every voxel's 12 CRSS values are equal (uploaded uniform and kept so by a
   // hardening law whose matrix rows and Voce constants are slip-independent)
   bool crss_uniform = false;
@@ -236,6 +237,15 @@ static void fill_host_const(Grid* g, const double edot[5], double dt) {
   C.nzh = g->nzh;
 }
 
+// max(err_stress, err_strain_rate) of the last record over the same five
+// records earlier
+static double err_ratio_5(const IterRecord* rec, int it) {
+  if (it < 6) return 1.0;
+  const double a = std::max(rec[it - 1].err_stress, rec[it - 1].err_strain_rate);
+  const double b = std::max(rec[it - 6].err_stress, rec[it - 6].err_strain_rate);
+  return b > 0 ? a / b : 1.0;
+}
+
 // Bracket one kernel launch with events when profiling is on.
 static inline void kbegin(Grid* g, int k) {
   if (g->profile) {
@@ -264,19 +274,20 @@ static int run_fft_green(Grid* g, int prec, const int* done) {
   void* const* tw = prec == 0 ? g->tw64 : g->tw32;
   const void* gc = prec == 0 ? (const void*)g->dG64 : (const void*)g->dG32;
   kbegin(g, K_Z_R2C);
-  HIP_TRY(launch_z_r2c(prec, g->phi, g->spec, g->geom, g->plan[2], tw[2], done, g->st));
+  HIP_TRY(launch_z_r2c(prec, g->phi, g->spec, g->geom, g->plan[2], tw[2], g->fftB[prec][0], done, g->st));
   kend(g, K_Z_R2C);
   kbegin(g, K_Y_FWD);
-  HIP_TRY(launch_y(prec, g->spec, g->geom, g->plan[1], tw[1], -1, done, g->st));
+  HIP_TRY(launch_y(prec, g->spec, g->geom, g->plan[1], tw[1], -1, g->fftB[prec][1], done, g->st));
   kend(g, K_Y_FWD);
   kbegin(g, K_X_GREEN);
-  HIP_TRY(launch_x_green(prec, g->spec, g->geom, g->plan[0], tw[0], gc, done, g->st));
+  HIP_TRY(launch_x_green(prec, g->spec, g->geom, g->plan[0], tw[0], gc, g->fftB[prec][2], done, g->st));
   kend(g, K_X_GREEN);
   kbegin(g, K_Y_INV);
-  HIP_TRY(launch_y(prec, g->spec, g->geom, g->plan[1], tw[1], +1, done, g->st));
+  HIP_TRY(launch_y(prec, g->spec, g->geom, g->plan[1], tw[1], +1, g->fftB[prec][1], done, g->st));
   kend(g, K_Y_INV);
   kbegin(g, K_Z_C2R);
-  HIP_TRY(launch_z_c2r(prec, g->spec, g->dtil, g->geom, g->plan[2], tw[2], 1.0 / (double)g->n, done, g->st));
+  HIP_TRY(launch_z_c2r(prec, g->spec, g->dtil, g->geom, g->plan[2], tw[2], 1.0 / (double)g->n, g->fftB[prec][0], done,
+                       g->st));
   kend(g, K_Z_C2R);
   g->dtil_prec = prec;
   return VPFFT_OK;
@@ -623,6 +634,7 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
     info->nr_steps_last = last.nr_steps;
     info->nr_steps_total = nr_total;
     info->nr_wave_steps_total = wave_total;
+    info->err_ratio_5 = err_ratio_5(g->hrec, it);
     float ms = 0;
     (void)hipEventElapsedTime(&ms, g->ev[0], g->ev[1]);
     info->ms_prologue = ms;
@@ -1074,6 +1086,7 @@ static int batch_step(Batch* b, Grid** gs, int count, const double* edot, int ma
       info->nr_steps_last = last.nr_steps;
       info->nr_steps_total = nr_total;
       info->nr_wave_steps_total = wave_total;
+      info->err_ratio_5 = err_ratio_5(g->hrec, it);
       info->ms_prologue = ms[0];
       info->ms_loop = ms[1];
       info->ms_epilogue = ms[2];
@@ -1521,6 +1534,24 @@ int vpfft_grid_restore_state(void* grid) {
   return VPFFT_OK;
 }
 
+int vpfft_grid_set_block_shape(void* grid, int precision, int pass, int block) {
+  Grid* g = as_grid(grid);
+  if (!g || pass < 0 || pass > 2 || (precision != VPFFT_FP64 && precision != VPFFT_MIXED) || block < 0)
+    return set_error(VPFFT_E_INVALID, "bad argument");
+  if (block > 0) {
+    int cand[8];
+    const int nc = vpfft::pass_candidates((vpfft::FftPass)pass, cand, 8);
+    bool listed = false;
+    for (int c = 0; c < nc; c++) listed = listed || cand[c] == block;
+    if (!listed || !vpfft::pass_shape(precision, (vpfft::FftPass)pass, g->geom, block).ok)
+      return set_error(VPFFT_E_INVALID, "block shape %d not available for pass %d of this grid", block, pass);
+  }
+  g->fftB[precision][pass] = block;
+  g->gkey.prec = -1;  // recapture the loop graph
+  return VPFFT_OK;
+}
+
 }  // extern "C"
+
 
 

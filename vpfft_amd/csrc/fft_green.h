@@ -77,16 +77,17 @@ hipError_t launch_y_group(int prec, int B, const FftGroup& grp, long blocks, siz
 hipError_t launch_x_green_group(int prec, int B, int E, const FftGroup& grp, long blocks, size_t lds,
                                 hipStream_t st);
 
-// prec: 0 = FP64 storage/compute, 1 = FP32.  done: optional device flag; a
+// prec: 0 = FP64 storage/compute, 1 = FP32.  B: block shape (pencils per
+// block, see PassShape; 0 = the default).  done: optional device flag; a
 // kernel whose flag is set returns at entry (graph-captured iterations after
 // convergence do nothing).
-hipError_t launch_z_r2c(int prec, const void* f, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
+hipError_t launch_z_r2c(int prec, const void* f, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int B,
                         const int* done, hipStream_t st);
 hipError_t launch_z_c2r(int prec, const void* s, void* f, const FftGeom& g, const AxisPlan& ap, const void* tw,
-                        double scale, const int* done, hipStream_t st);
-hipError_t launch_y(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int sign,
+                        double scale, int B, const int* done, hipStream_t st);
+hipError_t launch_y(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int sign, int B,
                     const int* done, hipStream_t st);
 hipError_t launch_x_green(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, const void* gc,
-                          const int* done, hipStream_t st);
+                          int B, const int* done, hipStream_t st);
 
 }  // namespace vpfft

@@ -50,10 +50,14 @@ template <typename T> __device__ __forceinline__ typename C2<T>::t cmake(T a, T 
 }
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { C r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+// fused multiply-adds written out (contraction is off in this file)
+__device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
   C r;
-  r.x = a.x * b.x - a.y * b.y;
-  r.y = a.x * b.y + a.y * b.x;
+  r.x = fmad(a.x, b.x, -(a.y * b.y));
+  r.y = fmad(a.x, b.y, a.y * b.x);
   return r;
 }
 // multiply by (s * i), s = +-1
@@ -91,8 +95,8 @@ __device__ __forceinline__ void dft_odd(typename C2<T>::t* v, int sign) {
     for (int r = 1; r < R; r++) {
       const int t = (r * k) % R;
       const T c = (T)cs[t], s = (T)(sign * sn[t]);
-      re += v[r].x * c - v[r].y * s;
-      im += v[r].x * s + v[r].y * c;
+      re = fmad(-v[r].y, s, fmad(v[r].x, c, re));
+      im = fmad(v[r].y, c, fmad(v[r].x, s, im));
     }
     out[k].x = re;
     out[k].y = im;
@@ -401,24 +405,24 @@ __device__ __forceinline__ void green_apply(typename C2<T>::t ph[5], int ix, int
     for (int r = 0; r < 6; r++) {
       T acc = 0;
 #pragma unroll
-      for (int q = 0; q < 6; q++) acc += G.Kc[r][q] * mono[q];
+      for (int q = 0; q < 6; q++) acc = fmad(G.Kc[r][q], mono[q], acc);
       K[r] = acc;
     }
     // K = [[a f e],[f b d],[e d c]]
     const T a = K[0], b = K[1], c = K[2], d = K[3], e = K[4], f = K[5];
-    const T A00 = b * c - d * d, A11 = a * c - e * e, A22 = a * b - f * f;
-    const T A12 = e * f - a * d, A02 = f * d - b * e, A01 = d * e - c * f;
-    const T det = a * A00 + f * A01 + e * A02;
-    const T p0 = A00 * k0 + A01 * k1 + A02 * k2;
-    const T p1 = A01 * k0 + A11 * k1 + A12 * k2;
-    const T p2 = A02 * k0 + A12 * k1 + A22 * k2;
-    const T sk = k0 * p0 + k1 * p1 + k2 * p2;
+    const T A00 = fmad(b, c, -(d * d)), A11 = fmad(a, c, -(e * e)), A22 = fmad(a, b, -(f * f));
+    const T A12 = fmad(e, f, -(a * d)), A02 = fmad(f, d, -(b * e)), A01 = fmad(d, e, -(c * f));
+    const T det = fmad(e, A02, fmad(f, A01, a * A00));
+    const T p0 = fmad(A02, k2, fmad(A01, k1, A00 * k0));
+    const T p1 = fmad(A12, k2, fmad(A11, k1, A01 * k0));
+    const T p2 = fmad(A22, k2, fmad(A12, k1, A02 * k0));
+    const T sk = fmad(k2, p2, fmad(k1, p1, k0 * p0));
     const T inv = (T)1 / (sk * det);
     // N = (adj * sk - p p^T) / (sk det)
-    const T N00 = (A00 * sk - p0 * p0) * inv, N11 = (A11 * sk - p1 * p1) * inv,
-            N22 = (A22 * sk - p2 * p2) * inv;
-    const T N12 = (A12 * sk - p1 * p2) * inv, N02 = (A02 * sk - p0 * p2) * inv,
-            N01 = (A01 * sk - p0 * p1) * inv;
+    const T N00 = fmad(A00, sk, -(p0 * p0)) * inv, N11 = fmad(A11, sk, -(p1 * p1)) * inv,
+            N22 = fmad(A22, sk, -(p2 * p2)) * inv;
+    const T N12 = fmad(A12, sk, -(p1 * p2)) * inv, N02 = fmad(A02, sk, -(p0 * p2)) * inv,
+            N01 = fmad(A01, sk, -(p0 * p1)) * inv;
     const T s2 = (T)0.70710678118654752440, s6 = (T)0.40824829046386301637;
     // u = V^T phi
     C u0, u1, u2;
@@ -429,12 +433,12 @@ __device__ __forceinline__ void green_apply(typename C2<T>::t ph[5], int ix, int
     u2.x = (T)2 * s6 * k2 * ph[1].x + s2 * k1 * ph[2].x + s2 * k0 * ph[3].x;
     u2.y = (T)2 * s6 * k2 * ph[1].y + s2 * k1 * ph[2].y + s2 * k0 * ph[3].y;
     C w0, w1, w2;
-    w0.x = N00 * u0.x + N01 * u1.x + N02 * u2.x;
-    w0.y = N00 * u0.y + N01 * u1.y + N02 * u2.y;
-    w1.x = N01 * u0.x + N11 * u1.x + N12 * u2.x;
-    w1.y = N01 * u0.y + N11 * u1.y + N12 * u2.y;
-    w2.x = N02 * u0.x + N12 * u1.x + N22 * u2.x;
-    w2.y = N02 * u0.y + N12 * u1.y + N22 * u2.y;
+    w0.x = fmad(N02, u2.x, fmad(N01, u1.x, N00 * u0.x));
+    w0.y = fmad(N02, u2.y, fmad(N01, u1.y, N00 * u0.y));
+    w1.x = fmad(N12, u2.x, fmad(N11, u1.x, N01 * u0.x));
+    w1.y = fmad(N12, u2.y, fmad(N11, u1.y, N01 * u0.y));
+    w2.x = fmad(N22, u2.x, fmad(N12, u1.x, N02 * u0.x));
+    w2.y = fmad(N22, u2.y, fmad(N12, u1.y, N02 * u0.y));
     const T wt = (sp || spn) ? (T)0.5 : (T)1;
     out[0].x = wt * s2 * (-k0 * w0.x + k1 * w1.x);
     out[0].y = wt * s2 * (-k0 * w0.y + k1 * w1.y);
@@ -455,11 +459,11 @@ __device__ __forceinline__ void green_apply(typename C2<T>::t ph[5], int ix, int
       T re = 0, im = 0;
 #pragma unroll
       for (int j = 0; j < 5; j++) {
-        re += G.L0inv[j * 5 + i] * ph[j].x;
-        im += G.L0inv[j * 5 + i] * ph[j].y;
+        re = fmad(G.L0inv[j * 5 + i], ph[j].x, re);
+        im = fmad(G.L0inv[j * 5 + i], ph[j].y, im);
       }
-      out[i].x += wt * re;
-      out[i].y += wt * im;
+      out[i].x = fmad(wt, re, out[i].x);
+      out[i].y = fmad(wt, im, out[i].y);
     }
   }
 #pragma unroll
@@ -537,8 +541,8 @@ __device__ __forceinline__ void x_green_body(typename C2<T>::t* __restrict__ s, 
   }
 }
 
-template <typename T, int B, int E>
-__global__ void __launch_bounds__(512) k_x_green(typename C2<T>::t* __restrict__ s, FftGeom g, AxisPlan ap,
+template <typename T, int B, int E, int NT>
+__global__ void __launch_bounds__(NT) k_x_green(typename C2<T>::t* __restrict__ s, FftGeom g, AxisPlan ap,
                                                  const typename C2<T>::t* __restrict__ tw,
                                                  const GreenConst<T>* __restrict__ gc, const int* __restrict__ done) {
   if (done && *done) return;
@@ -559,6 +563,16 @@ __device__ __forceinline__ int group_member(const FftGroup& grp, long b, long* l
   }
   *local = b - uni(grp.off[lo]);
   return uni(grp.ids[lo]);
+}
+
+// the fields lds_fft reads (the radix array itself is not copied: an indexed
+// local array would live in scratch)
+__device__ __forceinline__ AxisPlan uni_plan(const AxisPlan& a) {
+  AxisPlan r;
+  r.n = uni(a.n);
+  r.nstages = uni(a.nstages);
+  r.code = (unsigned long long)uni((long)a.code);
+  return r;
 }
 
 __device__ __forceinline__ FftGeom uni(const FftGeom& g) {
@@ -582,7 +596,7 @@ __global__ void __launch_bounds__(256) k_z_r2c_b(FftGroup grp) {
   const int* done = uni(J->done);
   if (done && *done) return;
   const FftGeom g = uni(J->g);
-  const AxisPlan& ap = J->az;  // read per stage, see lds_fft
+  const AxisPlan ap = uni_plan(J->az);
   z_r2c_body<T, B>(guni((const T*)J->field), guni((C*)J->spec), g, ap, guni((const C*)J->tw[2]), local);
 }
 template <typename T, int B>
@@ -593,7 +607,7 @@ __global__ void __launch_bounds__(256) k_z_c2r_b(FftGroup grp) {
   const int* done = uni(J->done);
   if (done && *done) return;
   const FftGeom g = uni(J->g);
-  const AxisPlan& ap = J->az;  // read per stage, see lds_fft
+  const AxisPlan ap = uni_plan(J->az);
   z_c2r_body<T, B>(guni((const C*)J->spec), guni((T*)J->dtil), g, ap, guni((const C*)J->tw[2]),
                    (T)(1.0 / (double)g.n), local);
 }
@@ -605,18 +619,18 @@ __global__ void __launch_bounds__(256) k_y_b(FftGroup grp, int sign) {
   const int* done = uni(J->done);
   if (done && *done) return;
   const FftGeom g = uni(J->g);
-  const AxisPlan& ap = J->ay;  // read per stage, see lds_fft
+  const AxisPlan ap = uni_plan(J->ay);
   y_body<T, B>(guni((C*)J->spec), g, ap, guni((const C*)J->tw[1]), sign, local);
 }
-template <typename T, int B, int E>
-__global__ void __launch_bounds__(512) k_x_green_b(FftGroup grp) {
+template <typename T, int B, int E, int NT>
+__global__ void __launch_bounds__(NT) k_x_green_b(FftGroup grp) {
   using C = typename C2<T>::t;
   long local;
   const FftJob* J = grp.jobs + group_member(grp, blockIdx.x, &local);
   const int* done = uni(J->done);
   if (done && *done) return;
   const FftGeom g = uni(J->g);
-  const AxisPlan& ap = J->ax;  // read per stage, see lds_fft
+  const AxisPlan ap = uni_plan(J->ax);
   x_green_body<T, B, E>(guni((C*)J->spec), g, ap, guni((const C*)J->tw[0]), uni((const GreenConst<T>*)J->gc), local);
 }
 
@@ -657,111 +671,112 @@ static int pick_run(int n, int budget) {
 }
 
 template <typename T>
-static hipError_t launch_z_r2c_t(const T* f, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
+static hipError_t launch_z_r2c_t(const T* f, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int Bq,
                                  const int* done, hipStream_t st) {
   using C = typename C2<T>::t;
-  const int B = pick_rows(g.nz, sizeof(T) == 8 ? 1536 : 3072);
-  const unsigned grid = (unsigned)((g.nrows + B - 1) / B);
-  const size_t lds = 2 * sizeof(C) * 3 * B * g.nz;
-  switch (B) {
-#define CASE(BB) case BB: hipLaunchKernelGGL((k_z_r2c<T, BB>), dim3(grid), dim3(256), lds, st, f, (C*)s, g, ap, (const C*)tw, done); break;
-    CASE(1) CASE(2) CASE(4) CASE(8) default: CASE(16)
+  const PassShape ps = pass_shape(sizeof(T) == 8 ? 0 : 1, kPassZ, g, Bq);
+  if (!ps.ok) return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)ps.blocks;
+  switch (ps.B) {
+#define CASE(BB) case BB: hipLaunchKernelGGL((k_z_r2c<T, BB>), dim3(grid), dim3(256), ps.lds, st, f, (C*)s, g, ap, (const C*)tw, done); break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16)
 #undef CASE
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 template <typename T>
 static hipError_t launch_z_c2r_t(const void* s, T* f, const FftGeom& g, const AxisPlan& ap, const void* tw,
-                                 double scale, const int* done, hipStream_t st) {
+                                 double scale, int Bq, const int* done, hipStream_t st) {
   using C = typename C2<T>::t;
-  const int B = pick_rows(g.nz, sizeof(T) == 8 ? 1536 : 3072);
-  const unsigned grid = (unsigned)((g.nrows + B - 1) / B);
-  const size_t lds = 2 * sizeof(C) * 3 * B * g.nz;
-  switch (B) {
-#define CASE(BB) case BB: hipLaunchKernelGGL((k_z_c2r<T, BB>), dim3(grid), dim3(256), lds, st, (const C*)s, f, g, ap, (const C*)tw, (T)scale, done); break;
-    CASE(1) CASE(2) CASE(4) CASE(8) default: CASE(16)
+  const PassShape ps = pass_shape(sizeof(T) == 8 ? 0 : 1, kPassZ, g, Bq);
+  if (!ps.ok) return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)ps.blocks;
+  switch (ps.B) {
+#define CASE(BB) case BB: hipLaunchKernelGGL((k_z_c2r<T, BB>), dim3(grid), dim3(256), ps.lds, st, (const C*)s, f, g, ap, (const C*)tw, (T)scale, done); break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16)
 #undef CASE
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 template <typename T>
-static hipError_t launch_y_t(void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int sign,
+static hipError_t launch_y_t(void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int sign, int Bq,
                              const int* done, hipStream_t st) {
   using C = typename C2<T>::t;
-  const int B = pick_run(g.ny, sizeof(T) == 8 ? 2048 : 4096);
-  const int runs = (g.nzh + B - 1) / B;
-  const unsigned grid = (unsigned)(5L * g.nx * runs);
-  const size_t lds = 2 * sizeof(C) * B * g.ny;
-  switch (B) {
-#define CASE(BB) case BB: hipLaunchKernelGGL((k_y<T, BB>), dim3(grid), dim3(256), lds, st, (C*)s, g, ap, (const C*)tw, sign, done); break;
-    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) default: CASE(64)
+  const PassShape ps = pass_shape(sizeof(T) == 8 ? 0 : 1, kPassY, g, Bq);
+  if (!ps.ok) return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)ps.blocks;
+  switch (ps.B) {
+#define CASE(BB) case BB: hipLaunchKernelGGL((k_y<T, BB>), dim3(grid), dim3(256), ps.lds, st, (C*)s, g, ap, (const C*)tw, sign, done); break;
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
 #undef CASE
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 template <typename T>
 static hipError_t launch_x_green_t(void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
-                                   const void* gc, const int* done, hipStream_t st) {
+                                   const void* gc, int Bq, const int* done, hipStream_t st) {
   using C = typename C2<T>::t;
-  // runs of at least 128 bytes along kz (8 complex doubles, 16 complex
-  // floats); 512 threads keep 5 * E complex values per thread in registers
-  const int minrun = sizeof(T) == 8 ? 8 : 16;
-  int B = pick_run(g.nx, 2048);
-  if (B < minrun) B = minrun;
-  const int runs = (g.nzh + B - 1) / B;
-  const unsigned grid = (unsigned)((long)g.ny * runs);
-  const size_t lds = 2 * sizeof(C) * B * g.nx;
-  const int cnt = B * g.nx;
-  const int E = (cnt + 511) / 512;
+  const PassShape ps = pass_shape(sizeof(T) == 8 ? 0 : 1, kPassX, g, Bq);
+  if (!ps.ok) return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)ps.blocks;
+  const int E = ps.E;
   const GreenConst<T>* G = (const GreenConst<T>*)gc;
-#define LAUNCH(BB, EE) hipLaunchKernelGGL((k_x_green<T, BB, EE>), dim3(grid), dim3(512), lds, st, (C*)s, g, ap, (const C*)tw, G, done)
-#define EC(BB)                         \
-  case BB:                             \
-    if (E <= 1) LAUNCH(BB, 1);         \
-    else if (E <= 2) LAUNCH(BB, 2);    \
-    else if (E <= 4) LAUNCH(BB, 4);    \
-    else LAUNCH(BB, 8);                \
+#define LAUNCH(BB, EE, TT) hipLaunchKernelGGL((k_x_green<T, BB, EE, TT>), dim3(grid), dim3(TT), ps.lds, st, (C*)s, g, ap, (const C*)tw, G, done)
+#define EC(BB, TT)                              \
+  case BB:                                      \
+    if (E == 1) LAUNCH(BB, 1, TT);              \
+    else if (E == 2) LAUNCH(BB, 2, TT);         \
+    else if (E == 4) LAUNCH(BB, 4, TT);         \
+    else LAUNCH(BB, 8, TT);                     \
     break;
-  switch (B) {
-    EC(8) EC(16) EC(32) default: EC(64)
+  switch (ps.B) {
+    EC(4, 256) EC(8, 512) EC(16, 512) EC(32, 512) EC(64, 512)
+    default: return hipErrorInvalidValue;
   }
 #undef EC
 #undef LAUNCH
   return hipGetLastError();
 }
 
-hipError_t launch_z_r2c(int prec, const void* f, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
+hipError_t launch_z_r2c(int prec, const void* f, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int B,
                         const int* done, hipStream_t st) {
-  return prec == 0 ? launch_z_r2c_t<double>((const double*)f, s, g, ap, tw, done, st)
-                   : launch_z_r2c_t<float>((const float*)f, s, g, ap, tw, done, st);
+  return prec == 0 ? launch_z_r2c_t<double>((const double*)f, s, g, ap, tw, B, done, st)
+                   : launch_z_r2c_t<float>((const float*)f, s, g, ap, tw, B, done, st);
 }
 hipError_t launch_z_c2r(int prec, const void* s, void* f, const FftGeom& g, const AxisPlan& ap, const void* tw,
-                        double scale, const int* done, hipStream_t st) {
-  return prec == 0 ? launch_z_c2r_t<double>(s, (double*)f, g, ap, tw, scale, done, st)
-                   : launch_z_c2r_t<float>(s, (float*)f, g, ap, tw, scale, done, st);
+                        double scale, int B, const int* done, hipStream_t st) {
+  return prec == 0 ? launch_z_c2r_t<double>(s, (double*)f, g, ap, tw, scale, B, done, st)
+                   : launch_z_c2r_t<float>(s, (float*)f, g, ap, tw, scale, B, done, st);
 }
-hipError_t launch_y(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int sign,
+hipError_t launch_y(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int sign, int B,
                     const int* done, hipStream_t st) {
-  return prec == 0 ? launch_y_t<double>(s, g, ap, tw, sign, done, st) : launch_y_t<float>(s, g, ap, tw, sign, done, st);
+  return prec == 0 ? launch_y_t<double>(s, g, ap, tw, sign, B, done, st)
+                   : launch_y_t<float>(s, g, ap, tw, sign, B, done, st);
 }
 hipError_t launch_x_green(int prec, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, const void* gc,
-                          const int* done, hipStream_t st) {
-  return prec == 0 ? launch_x_green_t<double>(s, g, ap, tw, gc, done, st)
-                   : launch_x_green_t<float>(s, g, ap, tw, gc, done, st);
+                          int B, const int* done, hipStream_t st) {
+  return prec == 0 ? launch_x_green_t<double>(s, g, ap, tw, gc, B, done, st)
+                   : launch_x_green_t<float>(s, g, ap, tw, gc, B, done, st);
 }
 
 // ---- batched launches ------------------------------------------------------
 static constexpr size_t kMaxLds = 65536;
 
+// threads of an x-pass block: 256 for runs of up to 4 columns, else 512
+static int x_threads(int b) { return b <= 4 ? 256 : 512; }
+
 int pass_candidates(FftPass pass, int* out, int cap) {
   static const int z[] = {1, 2, 4, 8, 16};
   static const int y[] = {1, 2, 4, 8, 16, 32, 64};
-  static const int x[] = {8, 16, 32, 64};
+  static const int x[] = {4, 8, 16, 32, 64};
   const int* src = pass == kPassZ ? z : (pass == kPassY ? y : x);
-  const int cnt = pass == kPassZ ? 5 : (pass == kPassY ? 7 : 4);
+  const int cnt = pass == kPassZ ? 5 : (pass == kPassY ? 7 : 5);
   int k = 0;
   for (; k < cnt && k < cap; k++) out[k] = src[k];
   return k;
@@ -771,24 +786,27 @@ PassShape pass_shape(int prec, FftPass pass, const FftGeom& g, int B) {
   const size_t csz = prec == 0 ? 16 : 8;
   PassShape ps{0, 1, 0, 0, false};
   if (pass == kPassZ) {
-    ps.B = B > 0 ? B : pick_rows(g.nz, prec == 0 ? 1536 : 3072);
+    ps.B = B > 0 ? B : pick_rows(g.nz, prec == 0 ? 768 : 1536);
     ps.blocks = (g.nrows + ps.B - 1) / ps.B;
     ps.lds = 2 * csz * 3 * ps.B * g.nz;
   } else if (pass == kPassY) {
-    ps.B = B > 0 ? B : pick_run(g.ny, prec == 0 ? 2048 : 4096);
+    ps.B = B > 0 ? B : pick_run(g.ny, prec == 0 ? 1024 : 2048);
     ps.blocks = 5L * g.nx * ((g.nzh + ps.B - 1) / ps.B);
     ps.lds = 2 * csz * ps.B * g.ny;
   } else {
     int b = B;
     if (b <= 0) {
-      const int minrun = prec == 0 ? 8 : 16;
-      b = pick_run(g.nx, 2048);
+      // runs of at least 64 bytes along kz; measured at 256^3: 4 columns of
+      // complex doubles in 256-thread blocks, 8 of complex floats in 512
+      const int minrun = prec == 0 ? 4 : 8;
+      b = pick_run(g.nx, prec == 0 ? 1024 : 2048);
       if (b < minrun) b = minrun;
     }
     ps.B = b;
     ps.blocks = (long)g.ny * ((g.nzh + b - 1) / b);
     ps.lds = 2 * csz * b * g.nx;
-    ps.E = (b * g.nx + 511) / 512;
+    const int nt = x_threads(b);
+    ps.E = (b * g.nx + nt - 1) / nt;
     if (ps.E > 8) return ps;  // register budget of the x pass
     ps.E = ps.E <= 1 ? 1 : (ps.E <= 2 ? 2 : (ps.E <= 4 ? 4 : 8));
   }
@@ -828,17 +846,17 @@ static hipError_t y_group_t(int B, const FftGroup& grp, long blocks, size_t lds,
 }
 template <typename T>
 static hipError_t x_group_t(int B, int E, const FftGroup& grp, long blocks, size_t lds, hipStream_t st) {
-#define LAUNCH(BB, EE) hipLaunchKernelGGL((k_x_green_b<T, BB, EE>), dim3((unsigned)blocks), dim3(512), lds, st, grp)
-#define EC(BB)                                  \
+#define LAUNCH(BB, EE, TT) hipLaunchKernelGGL((k_x_green_b<T, BB, EE, TT>), dim3((unsigned)blocks), dim3(TT), lds, st, grp)
+#define EC(BB, TT)                              \
   case BB:                                      \
-    if (E == 1) LAUNCH(BB, 1);                  \
-    else if (E == 2) LAUNCH(BB, 2);             \
-    else if (E == 4) LAUNCH(BB, 4);             \
-    else if (E == 8) LAUNCH(BB, 8);             \
+    if (E == 1) LAUNCH(BB, 1, TT);              \
+    else if (E == 2) LAUNCH(BB, 2, TT);         \
+    else if (E == 4) LAUNCH(BB, 4, TT);         \
+    else if (E == 8) LAUNCH(BB, 8, TT);         \
     else return hipErrorInvalidValue;           \
     break;
   switch (B) {
-    EC(8) EC(16) EC(32) EC(64)
+    EC(4, 256) EC(8, 512) EC(16, 512) EC(32, 512) EC(64, 512)
     default: return hipErrorInvalidValue;
   }
 #undef EC

@@ -166,11 +166,16 @@ extern "C" int vpfft_serve(const vpfft_serve_config* cfg, const vpfft_request* r
     steps_run++;
     if (rc && rc != VPFFT_E_DEGENERATE && rc != VPFFT_E_NONFINITE) break;  // the engine failed: stop serving
     rc = VPFFT_OK;
-    // mixed steps that did not converge: rerun them in FP64 from the saved state
+    for (int j = 0; j < m; j++) {  // work done, counted per attempt
+      res[active[j].req].iterations += infos[j].iterations;
+      voxel_iters += active[j].voxels * infos[j].iterations;
+    }
+    // mixed steps that ended unconverged and stalled: rerun them in FP64 from the saved state
     if (fallback) {
       std::vector<int> redo;
       for (int j = 0; j < m; j++)
-        if (infos[j].status == VPFFT_OK && !infos[j].converged) redo.push_back(j);
+        if (infos[j].status == VPFFT_OK && !infos[j].converged && infos[j].err_ratio_5 >= cfg->stall_ratio)
+          redo.push_back(j);
       if (!redo.empty()) {
         std::vector<void*> rg;
         std::vector<double> red, rdt;
@@ -191,6 +196,8 @@ extern "C" int vpfft_serve(const vpfft_serve_config* cfg, const vpfft_request* r
           vpfft_grid_set_precision(gs[j], VPFFT_MIXED);
           std::memcpy(&avg[5 * j], &ravg[5 * k], 5 * sizeof(double));
           infos[j] = rinf[k];
+          res[active[j].req].iterations += rinf[k].iterations;
+          voxel_iters += active[j].voxels * rinf[k].iterations;
           if (rrc && infos[j].status == VPFFT_OK) infos[j].status = rrc;
           res[active[j].req].fp64_reruns++;
           reruns_total++;
@@ -203,8 +210,6 @@ extern "C" int vpfft_serve(const vpfft_serve_config* cfg, const vpfft_request* r
       const vpfft_request& q = reqs[A.req];
       vpfft_request_result& r = res[A.req];
       if (q.avg_stress) std::memcpy(q.avg_stress + 5 * A.next, &avg[5 * j], 5 * sizeof(double));
-      r.iterations += infos[j].iterations;
-      voxel_iters += A.voxels * infos[j].iterations;
       if (infos[j].status == VPFFT_OK) {
         A.next++;
         r.increments_done = A.next;

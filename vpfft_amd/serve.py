@@ -47,6 +47,7 @@ class vpfft_serve_config(C.Structure):
         ("max_active", C.c_int),
         ("precision", C.c_int),
         ("fp64_fallback", C.c_int),
+        ("stall_ratio", C.c_double),
         ("max_iter", C.c_int),
         ("tol", C.c_double),
         ("tune", C.c_int),
@@ -75,7 +76,7 @@ def _struct_dict(s) -> dict:
 
 def serve(requests: Sequence[dict], hbm_budget_bytes: int = 64 << 30, max_active: int = 64,
           precision: int = MIXED, fp64_fallback: bool = True, max_iter: int = 100, tol: float = 1e-5,
-          tune: bool = True):
+          tune: bool = True, stall_ratio: float = 0.9):
     """Serve the requests to completion.  Returns (results, stats, avg_stress)
     where avg_stress[i] is request i's <sigma> per increment (k x 5)."""
     lib = load_library()
@@ -102,7 +103,7 @@ def serve(requests: Sequence[dict], hbm_budget_bytes: int = 64 << 30, max_active
         r.dt = float(q.get("dt", 0.005))
         r.avg_stress = o.ctypes.data_as(C.POINTER(C.c_double))
     cfg = vpfft_serve_config(int(hbm_budget_bytes), int(max_active), int(precision), 1 if fp64_fallback else 0,
-                             int(max_iter), float(tol), 1 if tune else 0)
+                             float(stall_ratio), int(max_iter), float(tol), 1 if tune else 0)
     res = (vpfft_request_result * max(count, 1))()
     st = vpfft_serve_stats()
     _check(lib.vpfft_serve(C.byref(cfg), reqs, count, res, C.byref(st)))
