@@ -631,6 +631,16 @@ __device__ __forceinline__ void dd_mul(double ah, double al, double bh, double b
   *rh = s;
 }
 
+// x / y, correctly rounded, from rinv = RN(1/y): q = RN(x rinv) is within an
+// ulp of x / y, the remainder x - q y is exact in one fma, and q + rem rinv
+// rounds to RN(x / y) (Markstein's theorem; normal range).  Three operations
+// instead of a full division when one divisor serves many quotients.
+__device__ __forceinline__ double div_cr(double x, double y, double rinv) {
+  const double q = x * rinv;
+  const double rem = __builtin_fma(-q, y, x);
+  return __builtin_fma(rem, rinv, q);
+}
+
 // x^n (n >= 0) rounded once from a double-double binary power.
 __device__ __forceinline__ double ipow_cr(double x, int n) {
 #pragma clang fp contract(off)
@@ -782,9 +792,10 @@ __device__ __forceinline__ void lu_fullpiv(const double Ac[25], const double bi[
         cp[k] = sw ? cp[j] : t;
         cp[j] = sw ? t : cp[j];
       }
+      const double rkk = 1.0 / A[k][k];  // the multipliers below are quotients by A[k][k] (div_cr)
 #pragma unroll
       for (int i = k + 1; i < 5; i++) {
-        const double f = A[i][k] / A[k][k];
+        const double f = div_cr(A[i][k], A[k][k], rkk);
         A[i][k] = f;
 #pragma unroll
         for (int j = k + 1; j < 5; j++) A[i][j] -= f * A[k][j];
@@ -814,9 +825,11 @@ __device__ __forceinline__ void lu_fullpiv(const double Ac[25], const double bi[
 __device__ __forceinline__ void taylor_nr(const double init[5], const double m[kSlips][5], const double tau[kSlips],
                                           const StepConst& C, double eps, int max_it, double out[5]) {
 #pragma clang fp contract(off)
-  double cur[5], nw[5], saved[5];
+  double cur[5], nw[5], saved[5], rtau[kSlips];
 #pragma unroll
   for (int i = 0; i < 5; i++) { cur[i] = init[i]; nw[i] = init[i]; saved[i] = 0.0; }
+#pragma unroll
+  for (int k = 0; k < kSlips; k++) rtau[k] = 1.0 / tau[k];  // quotients by tau via div_cr
   int rem = max_it;
   const double max_rss = C.imposed_max_rss;
   while (rem > 0) {
@@ -826,7 +839,7 @@ __device__ __forceinline__ void taylor_nr(const double init[5], const double m[k
       ok = true;
 #pragma unroll
       for (int k = 0; k < kSlips; k++) {
-        double r = dot5(m[k], cur) / tau[k];
+        double r = div_cr(dot5(m[k], cur), tau[k], rtau[k]);
         if (fabs(r) < 1e-10) r = 0.0;
         if (fabs(r) > max_rss) ok = false;
         rss[k] = r;
@@ -853,15 +866,13 @@ __device__ __forceinline__ void taylor_nr(const double init[5], const double m[k
 #pragma unroll
       for (int i = 0; i < 5; i++) res[i] += m[k][i] * g * rss[k];
       // J(i,j) and J(j,i) receive bit-identical increments (m_i m_j == m_j m_i),
-      // so the lower triangle is computed and mirrored; x / 1.0 == x exactly,
-      // so the division is skipped when tau is 1
-      const bool unit_tau = tau[k] == 1.0;
+      // so the lower triangle is computed and mirrored
 #pragma unroll
       for (int j = 0; j < 5; j++)
 #pragma unroll
         for (int i = j; i < 5; i++) {
           const double t = m[k][i] * m[k][j] * (double)n * g;
-          J[j * 5 + i] -= unit_tau ? t : t / tau[k];
+          J[j * 5 + i] -= div_cr(t, tau[k], rtau[k]);
         }
     }
 #pragma unroll
