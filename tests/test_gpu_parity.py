@@ -81,7 +81,8 @@ def test_taylor_nr_batch_matches_oracle(gpu, vpfft, oracle, nexp):
     assert np.array_equal(out, ref), rel_err(out, ref)
 
 
-@pytest.mark.parametrize("dims", [(16, 16, 16), (8, 12, 10), (12, 10, 15), (24, 18, 20), (32, 16, 28)])
+@pytest.mark.parametrize("dims", [(16, 16, 16), (8, 12, 10), (12, 10, 15), (24, 18, 20), (32, 16, 28), (4, 256, 10),
+                                  (6, 256, 256), (256, 4, 6)])
 def test_strain_rate_evolution_matches_oracle(gpu, vpfft, oracle, dims):
     """FFT + Green + inverse FFT (R2C/C2R with the special-plane operator
     G_eff) against the reference's C2C formulation."""

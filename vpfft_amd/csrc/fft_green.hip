@@ -373,6 +373,236 @@ __global__ void __launch_bounds__(256) k_y(typename C2<T>::t* __restrict__ s, Ff
   y_body<T, B>(s, g, ap, tw, sign, blockIdx.x);
 }
 
+// ---------------------------------------------------------------------------
+// Length-256 transforms in registers: 256 = 16 x 16.  Sixteen threads own a
+// pencil; thread t holds x[t + 16 j] (j = 0..15), runs a 16-point DFT over j
+// in registers, multiplies by W256^(t k1), and one exchange through LDS
+// (real parts, then imaginary parts) gives thread k1 the values of all t for
+// its k1; a second 16-point DFT yields X[k1 + 16 k2].  One LDS round trip
+// replaces the three of the generic Stockham passes.
+// ---------------------------------------------------------------------------
+// 16-point DFT of v in place (natural order), W = exp(sign 2 pi i / 16):
+// 4 DFT-4 over m of v[a + 4m], twiddles W^(a b), 4 DFT-4 over a.
+template <typename T>
+__device__ __forceinline__ void dft16(typename C2<T>::t v[16], int sign) {
+  using C = typename C2<T>::t;
+  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, h = (T)0.70710678118654752440;
+  C u[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    C w[4] = {v[a], v[a + 4], v[a + 8], v[a + 12]};
+    dft<T, 4>(w, sign);
+#pragma unroll
+    for (int b = 0; b < 4; b++) u[a][b] = w[b];
+  }
+  // W16^(ab) for ab = 1, 2, 3, 4, 6, 9 (a, b in 1..3)
+  const T sg = (T)sign;
+  const C w1 = cmake<T>(c1, sg * s1), w2 = cmake<T>(h, sg * h), w3 = cmake<T>(s1, sg * c1);
+  const C w6 = cmake<T>(-h, sg * h), w9 = cmake<T>(-c1, -sg * s1);
+  u[1][1] = cmul(u[1][1], w1);
+  u[1][2] = cmul(u[1][2], w2);
+  u[1][3] = cmul(u[1][3], w3);
+  u[2][1] = cmul(u[2][1], w2);
+  u[2][2] = cmul_si(u[2][2], sign);  // W^4 = sign i
+  u[2][3] = cmul(u[2][3], w6);
+  u[3][1] = cmul(u[3][1], w3);
+  u[3][2] = cmul(u[3][2], w6);
+  u[3][3] = cmul(u[3][3], w9);
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    C w[4] = {u[0][b], u[1][b], u[2][b], u[3][b]};
+    dft<T, 4>(w, sign);
+#pragma unroll
+    for (int c = 0; c < 4; c++) v[b + 4 * c] = w[c];
+  }
+}
+
+// Exchange through LDS: thread (p, t) holds V[k1] (k1 = 0..15) of pencil p;
+// afterwards thread (p, t) holds W[tt] = V_tt[t] for tt = 0..15.  buf: 4096
+// reals, laid out [k1][t][p] with p fastest so stores and loads are 128-byte
+// runs.  Ends with a barrier.
+template <typename T>
+__device__ __forceinline__ void exchange16(typename C2<T>::t v[16], T* buf, int p, int t) {
+  T re[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) buf[(k * 16 + t) * 16 + p] = v[k].x;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) re[k] = buf[(t * 16 + k) * 16 + p];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) buf[(k * 16 + t) * 16 + p] = v[k].y;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) v[k] = cmake<T>(re[k], buf[(t * 16 + k) * 16 + p]);
+  __syncthreads();
+}
+
+// y pass of length 256, in place: block = 16 consecutive kz columns of one
+// (component, x) plane, 16 threads per column.
+template <typename T>
+__global__ void __launch_bounds__(256) k_y256(typename C2<T>::t* __restrict__ s, FftGeom g,
+                                              const typename C2<T>::t* __restrict__ tw, int sign,
+                                              const int* __restrict__ done) {
+  using C = typename C2<T>::t;
+  if (done && *done) return;
+  __shared__ T buf[4096];
+  const int runs = (g.nzh + 15) >> 4;
+  const int run = (int)(blockIdx.x % runs);
+  const long plane = blockIdx.x / runs;  // c * nx + x
+  const int p = threadIdx.x & 15, t = threadIdx.x >> 4;
+  const int kz = run * 16 + p;
+  const bool active = kz < g.nzh;
+  C* base = s + plane * (long)g.ny * g.nzp + kz;
+  C v[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) v[j] = active ? base[(long)(t + 16 * j) * g.nzp] : cmake<T>(0, 0);
+  dft16<T>(v, sign);
+#pragma unroll
+  for (int k = 1; k < 16; k++) {  // W256^(t k), sign as the transform
+    C w = tw[t * k];
+    if (sign > 0) w.y = -w.y;
+    v[k] = cmul(v[k], w);
+  }
+  exchange16<T>(v, buf, p, t);
+  dft16<T>(v, sign);
+  if (active) {
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) base[(long)(t + 16 * k2) * g.nzp] = v[k2];
+  }
+}
+
+// z passes of length 256 in the same 16 x 16 register form.  A pencil is
+// one (row, pair) of real components packed as f[2 pair] + i f[2 pair + 1]
+// (pair 2 holds component 4 alone); 16 pencils per block, thread t of a
+// pencil is the fastest index so a wave reads 128-byte runs along z.
+// LDS rows are padded to 257 reals so the transposing reads hit distinct banks.
+template <typename T>
+__device__ __forceinline__ void exchange16_zt(typename C2<T>::t v[16], T* buf, int p, int t) {
+  T re[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) buf[k * 257 + p * 16 + t] = v[k].x;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) re[k] = buf[t * 257 + p * 16 + k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) buf[k * 257 + p * 16 + t] = v[k].y;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) v[k] = cmake<T>(re[k], buf[t * 257 + p * 16 + k]);
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_z_r2c256(const T* __restrict__ f, typename C2<T>::t* __restrict__ s,
+                                                  FftGeom g, const typename C2<T>::t* __restrict__ tw,
+                                                  const int* __restrict__ done) {
+  using C = typename C2<T>::t;
+  if (done && *done) return;
+  __shared__ T buf[16 * 257];
+  const int t = threadIdx.x & 15, p = threadIdx.x >> 4;
+  const long P = (long)blockIdx.x * 16 + p;
+  const long row = P / 3;
+  const int pair = (int)(P - row * 3);
+  const bool active = row < g.nrows;
+  const T* fa = f + (long)(2 * pair) * g.n + row * 256;
+  const T* fb = f + (long)(2 * pair + 1) * g.n + row * 256;
+  C v[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const T a = active ? fa[t + 16 * j] : (T)0;
+    const T b = (active && pair < 2) ? fb[t + 16 * j] : (T)0;
+    v[j] = cmake<T>(a, b);
+  }
+  dft16<T>(v, -1);
+#pragma unroll
+  for (int k = 1; k < 16; k++) v[k] = cmul(v[k], tw[t * k]);
+  exchange16_zt<T>(v, buf, p, t);
+  dft16<T>(v, -1);  // thread t now holds Z[t + 16 k2]
+  // Hermitian unpack: X_a = (Z_k + conj Z_-k) / 2, X_b = -i/2 (Z_k - conj Z_-k)
+  // for k = t + 16 j <= 128; Z in natural order through LDS, real parts first
+  T zr[9], zi[9], nr[9], ni[9];
+#pragma unroll
+  for (int ph = 0; ph < 2; ph++) {
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) buf[p * 257 + t + 16 * k2] = ph == 0 ? v[k2].x : v[k2].y;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+      const int k = t + 16 * j;
+      const int kn = (256 - k) & 255;
+      const T a = k <= 128 ? buf[p * 257 + k] : (T)0;
+      const T b = k <= 128 ? buf[p * 257 + kn] : (T)0;
+      if (ph == 0) { zr[j] = a; nr[j] = b; } else { zi[j] = a; ni[j] = b; }
+    }
+    __syncthreads();
+  }
+  if (active) {
+    C* sa = s + (long)(2 * pair) * g.nc + row * g.nzp;
+    C* sb = s + (long)(2 * pair + 1) * g.nc + row * g.nzp;
+    const T h = (T)0.5;
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+      const int k = t + 16 * j;
+      if (k > 128) continue;
+      sa[k] = cmake<T>(h * (zr[j] + nr[j]), h * (zi[j] - ni[j]));
+      if (pair < 2) sb[k] = cmake<T>(h * (zi[j] + ni[j]), -h * (zr[j] - nr[j]));
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_z_c2r256(const typename C2<T>::t* __restrict__ s, T* __restrict__ f,
+                                                  FftGeom g, const typename C2<T>::t* __restrict__ tw, T scale,
+                                                  const int* __restrict__ done) {
+  using C = typename C2<T>::t;
+  if (done && *done) return;
+  __shared__ T buf[16 * 257];
+  const int t = threadIdx.x & 15, p = threadIdx.x >> 4;
+  const long P = (long)blockIdx.x * 16 + p;
+  const long row = P / 3;
+  const int pair = (int)(P - row * 3);
+  const bool active = row < g.nrows;
+  const C* sa = s + (long)(2 * pair) * g.nc + row * g.nzp;
+  const C* sb = s + (long)(2 * pair + 1) * g.nc + row * g.nzp;
+  C v[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const int k = t + 16 * j;
+    const int kk = k <= 128 ? k : 256 - k;  // Z_-k from the conjugates
+    C xa = cmake<T>(0, 0), xb = cmake<T>(0, 0);
+    if (active) {
+      xa = sa[kk];
+      if (pair < 2) xb = sb[kk];
+    }
+    if (kk == 0 || kk == 128) {  // C2R reads only the real part of the self-conjugate bins
+      xa.y = 0;
+      xb.y = 0;
+    }
+    // Z_k = X_a + i X_b ;  Z_-k = conj X_a + i conj X_b
+    v[j] = k <= 128 ? cmake<T>(xa.x - xb.y, xa.y + xb.x) : cmake<T>(xa.x + xb.y, xb.x - xa.y);
+  }
+  dft16<T>(v, +1);
+#pragma unroll
+  for (int k = 1; k < 16; k++) {
+    C w = tw[t * k];
+    w.y = -w.y;
+    v[k] = cmul(v[k], w);
+  }
+  exchange16_zt<T>(v, buf, p, t);
+  dft16<T>(v, +1);  // thread t holds z = t + 16 k2
+  if (active) {
+    T* fa = f + (long)(2 * pair) * g.n + row * 256;
+    T* fb = f + (long)(2 * pair + 1) * g.n + row * 256;
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) {
+      fa[t + 16 * k2] = v[k2].x * scale;
+      if (pair < 2) fb[t + 16 * k2] = v[k2].y * scale;
+    }
+  }
+}
+
 // Green operator applied to one frequency: phi <- G_eff(k) phi  (complex 5-vector).
 template <typename T>
 __device__ __forceinline__ void green_apply(typename C2<T>::t ph[5], int ix, int iy, int iz,
@@ -674,6 +904,11 @@ template <typename T>
 static hipError_t launch_z_r2c_t(const T* f, void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int Bq,
                                  const int* done, hipStream_t st) {
   using C = typename C2<T>::t;
+  if (g.nz == 256 && Bq == 0) {  // the register 16 x 16 form
+    const unsigned grid = (unsigned)((3 * g.nrows + 15) / 16);
+    hipLaunchKernelGGL((k_z_r2c256<T>), dim3(grid), dim3(256), 0, st, f, (C*)s, g, (const C*)tw, done);
+    return hipGetLastError();
+  }
   const PassShape ps = pass_shape(sizeof(T) == 8 ? 0 : 1, kPassZ, g, Bq);
   if (!ps.ok) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)ps.blocks;
@@ -690,6 +925,12 @@ template <typename T>
 static hipError_t launch_z_c2r_t(const void* s, T* f, const FftGeom& g, const AxisPlan& ap, const void* tw,
                                  double scale, int Bq, const int* done, hipStream_t st) {
   using C = typename C2<T>::t;
+  if (g.nz == 256 && Bq == 0) {  // the register 16 x 16 form
+    const unsigned grid = (unsigned)((3 * g.nrows + 15) / 16);
+    hipLaunchKernelGGL((k_z_c2r256<T>), dim3(grid), dim3(256), 0, st, (const C*)s, f, g, (const C*)tw, (T)scale,
+                       done);
+    return hipGetLastError();
+  }
   const PassShape ps = pass_shape(sizeof(T) == 8 ? 0 : 1, kPassZ, g, Bq);
   if (!ps.ok) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)ps.blocks;
@@ -706,6 +947,11 @@ template <typename T>
 static hipError_t launch_y_t(void* s, const FftGeom& g, const AxisPlan& ap, const void* tw, int sign, int Bq,
                              const int* done, hipStream_t st) {
   using C = typename C2<T>::t;
+  if (g.ny == 256 && Bq == 0) {  // the register 16 x 16 form
+    const unsigned grid = (unsigned)(5L * g.nx * ((g.nzh + 15) / 16));
+    hipLaunchKernelGGL((k_y256<T>), dim3(grid), dim3(256), 0, st, (C*)s, g, (const C*)tw, sign, done);
+    return hipGetLastError();
+  }
   const PassShape ps = pass_shape(sizeof(T) == 8 ? 0 : 1, kPassY, g, Bq);
   if (!ps.ok) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)ps.blocks;

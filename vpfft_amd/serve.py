@@ -76,9 +76,12 @@ def _struct_dict(s) -> dict:
 
 def serve(requests: Sequence[dict], hbm_budget_bytes: int = 64 << 30, max_active: int = 64,
           precision: int = MIXED, fp64_fallback: bool = True, max_iter: int = 100, tol: float = 1e-5,
-          tune: bool = True, stall_ratio: float = 0.9):
+          tune: bool = True, stall_ratio: float = 0.98):
     """Serve the requests to completion.  Returns (results, stats, avg_stress)
-    where avg_stress[i] is request i's <sigma> per increment (k x 5)."""
+    where avg_stress[i] is request i's <sigma> per increment (k x 5).
+    stall_ratio: a mixed step that ends unconverged is rerun in FP64 when its
+    error fell by less than 2% over its last five iterations (the FP32
+    rounding floor); slow but steady convergence (the same in FP64) is not."""
     lib = load_library()
     lib.vpfft_serve.argtypes = [C.POINTER(vpfft_serve_config), C.POINTER(vpfft_request), C.c_int,
                                 C.POINTER(vpfft_request_result), C.POINTER(vpfft_serve_stats)]
