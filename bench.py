@@ -27,10 +27,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
-# Algorithmic work per unit, matching the kernels' own arithmetic (the
-# derivation is in DESIGN.md, "Roofline accounting"):
-NR_FLOPS_FIXED = 1500.0       # per voxel per iteration, outside the Newton steps
-NR_FLOPS_PER_STEP = 1150.0    # per Newton step
+# Algorithmic work per unit: the FP64 arithmetic of the Newton kernel
+# (exponent 20, uniform CRSS) counted from its gfx950 assembly, an FMA as 2
+# flops, a division or square root as 1, compares and maxima not counted.
+# Outside the Newton loop: the crystal-frame maps Q(R) and Q(R)^-1, the
+# congruence P^T L0^-1 P, the rescale, the constitutive law, the lambda update
+# and the block sums.  Per Newton step: 12 resolved shears, the power law, the
+# crystal-frame Jacobian and residual, the 5x5 LDL^T solve and the map back.
+NR_FLOPS_FIXED = 1900.0       # per voxel per iteration, outside the Newton steps
+NR_FLOPS_PER_STEP = 730.0     # per Newton step
 # d~, sigma, lambda (40 B each), R (72 B) and one CRSS (8 B) read; sigma,
 # lambda, phi written (40 B each): the uniform-CRSS variant the workload runs
 # (every voxel's CRSS equal and kept equal by the hardening law)
@@ -360,6 +365,7 @@ def main():
         "nr_steps_per_voxel_iter": nr_steps_total / f64["vox_iters"],
         "nr_steps_max": max(i["max_nr_steps"] for i in f64["infos"]),
         "nr_lane_efficiency": nr_steps_total / sum(i["nr_wave_steps_total"] for i in f64["infos"]),
+        "taylor_steps_per_voxel": sum(i["taylor_steps_total"] for i in f64["infos"]) / (n * args.steps),
         "ms_prologue": sum(i["ms_prologue"] for i in f64["infos"]) / args.steps,
         "ms_per_step_host_launched": 1e3 * f64["profiled_wall"] / args.steps,
         "kernel_pass_same_bits": f64["profiled_same_bits"],

@@ -91,6 +91,7 @@ typedef struct {
   double nr_wave_steps_total;                  /* sum over iterations and 64-voxel waves of 64 x the wave's
                                                   largest step count: nr_steps_total / this = the Newton
                                                   loop's lane efficiency */
+  double taylor_steps_total;                   /* Newton steps of the Taylor initialisation, summed over voxels */
   double err_ratio_5;                          /* max(err_stress, err_strain_rate) of the last iteration
                                                   over the same five iterations earlier (1 with fewer):
                                                   near 1 = the fixed point has stalled */

@@ -2,8 +2,8 @@
 
 Every test calls the product through its C ABI (ctypes) and the oracle
 (oracle/libvpfft_oracle.so) on the same seeded inputs.  Tolerances are
-relative to the field's max magnitude; the north-star bar for the plain FP64
-form is 1e-8 with the same iteration count.
+relative to the field's max magnitude; the bar for the plain FP64 form is
+1e-8 with the same iteration count.
 """
 import numpy as np
 import pytest
@@ -57,6 +57,31 @@ def test_constrained_nr_batch_matches_oracle(gpu, vpfft, oracle, nexp):
     # size to 1e-3, and at the rounding floor only in absolute terms.
     bad = ~np.isclose(err, ref_err, rtol=1e-3, atol=1e-9)
     assert not bad.any(), list(zip(err[bad][:5], ref_err[bad][:5]))
+
+
+@pytest.mark.parametrize("nexp", [10, 20])
+def test_constrained_nr_batch_off_rotation_group(gpu, vpfft, oracle, nexp):
+    """Orientations that have left SO(3), as the epilogue's additive update
+    leaves them: the kernel's crystal-frame solve uses the exact inverse of
+    sigma -> dev(R^T sigma R), not its transpose."""
+    rng = np.random.default_rng(300 + nexp)
+    n = 64
+    R, crss, mat = _voxel_inputs(rng, n, vpfft, nexp)
+    R = R + 0.03 * rng.standard_normal(R.shape)  # |R^T R - I| ~ 0.05
+    L0 = _random_spd(rng).T.reshape(25)
+    edot = random_unit_edot(8)
+    s_in = rng.standard_normal((n, 5))
+    for v in range(n):
+        m = oracle.update_schmid(R[v])
+        s_in[v] /= oracle.max_rss(s_in[v], m, crss[v])
+    lam = s_in + 0.05 * rng.standard_normal((n, 5))
+    dt = 0.05 * rng.standard_normal((n, 5))
+    out, err, steps = vpfft.solve_constrained_batch(mat, R, crss, L0, edot, s_in, lam, dt)
+    ref = np.empty_like(out)
+    for v in range(n):
+        m = oracle.update_schmid(R[v])
+        ref[v], _ = oracle.constrained_nr(s_in[v], m, crss[v], mat, edot, lam[v], dt[v], L0)
+    assert rel_err(out, ref) < 1e-10, rel_err(out, ref)
 
 
 @pytest.mark.parametrize("nexp", [3, 10, 20])

@@ -85,6 +85,7 @@ class vpfft_step_info(C.Structure):
         ("ms_epilogue", C.c_double),
         ("max_nr_steps", C.c_double),
         ("nr_wave_steps_total", C.c_double),
+        ("taylor_steps_total", C.c_double),
         ("err_ratio_5", C.c_double),
     ]
 

@@ -635,6 +635,7 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
     info->nr_steps_total = nr_total;
     info->nr_wave_steps_total = wave_total;
     info->err_ratio_5 = err_ratio_5(g->hrec, it);
+    info->taylor_steps_total = g->hprol->taylor_steps;
     float ms = 0;
     (void)hipEventElapsedTime(&ms, g->ev[0], g->ev[1]);
     info->ms_prologue = ms;
@@ -1087,6 +1088,7 @@ static int batch_step(Batch* b, Grid** gs, int count, const double* edot, int ma
       info->nr_steps_total = nr_total;
       info->nr_wave_steps_total = wave_total;
       info->err_ratio_5 = err_ratio_5(g->hrec, it);
+      info->taylor_steps_total = g->hprol->taylor_steps;
       info->ms_prologue = ms[0];
       info->ms_loop = ms[1];
       info->ms_epilogue = ms[2];

@@ -40,10 +40,12 @@ struct StepConst {
 
 // Read-only view of per-step constants: loads through the constant address
 // space are invariant for the kernel, so uniform ones become scalar loads
-// the compiler may repeat instead of holding the values in registers.
+// the compiler may repeat instead of holding the values in registers.  The
+// pointer is rebuilt from its integer value: a generic pointer cast to the
+// constant space and straight back is folded back to generic.
 template <typename T>
 __device__ __forceinline__ const T* as_const_mem(const T* p) {
-  return (const T*)(const __attribute__((address_space(4))) T*)p;
+  return (const T*)(const __attribute__((address_space(4))) T*)(unsigned long)p;
 }
 
 // FCC {111}<110> systems in the reference's order (BurgerVectorBasis.h:16-72).
@@ -823,7 +825,8 @@ __device__ __forceinline__ void lu_fullpiv(const double Ac[25], const double bi[
 
 // Solvers::SolveConstitutiveEquations (Solvers.cpp:19-101), operation for operation.
 __device__ __forceinline__ void taylor_nr(const double init[5], const double m[kSlips][5], const double tau[kSlips],
-                                          const StepConst& C, double eps, int max_it, double out[5]) {
+                                          const StepConst& C, double eps, int max_it, double out[5],
+                                          int* steps_out = nullptr) {
 #pragma clang fp contract(off)
   double cur[5], nw[5], saved[5], rtau[kSlips];
 #pragma unroll
@@ -831,6 +834,7 @@ __device__ __forceinline__ void taylor_nr(const double init[5], const double m[k
 #pragma unroll
   for (int k = 0; k < kSlips; k++) rtau[k] = 1.0 / tau[k];  // quotients by tau via div_cr
   int rem = max_it;
+  int steps = 0;
   const double max_rss = C.imposed_max_rss;
   while (rem > 0) {
     double rss[kSlips];
@@ -852,6 +856,7 @@ __device__ __forceinline__ void taylor_nr(const double init[5], const double m[k
       if (rem < 0) {
 #pragma unroll
         for (int i = 0; i < 5; i++) out[i] = nw[i];
+        if (steps_out) *steps_out = steps;
         return;
       }
     } while (!ok);
@@ -885,6 +890,7 @@ __device__ __forceinline__ void taylor_nr(const double init[5], const double m[k
     for (int i = 0; i < 5; i++) saved[i] = cur[i];
     double delta[5];
     lu_fullpiv(J, res, delta);
+    steps++;
     double sum[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) {
@@ -898,6 +904,7 @@ __device__ __forceinline__ void taylor_nr(const double init[5], const double m[k
   }
 #pragma unroll
   for (int i = 0; i < 5; i++) out[i] = nw[i];
+  if (steps_out) *steps_out = steps;
 }
 
 }  // namespace faithful

@@ -183,9 +183,24 @@ __device__ __forceinline__ long uni(long x) {
   return (long)(((unsigned long)hi << 32) | lo);
 }
 template <typename P> __device__ __forceinline__ P* uni(P* p) { return (P*)uni((long)p); }
-// a uniform pointer read from a job table, known to address global memory
-template <typename P> __device__ __forceinline__ P* guni(P* p) {
-  return (P*)(__attribute__((address_space(1))) P*)uni(p);
+// A uniform pointer read from a job table, made directly in address space AS
+// (1 global, 4 constant) from its integer value.  A generic pointer cast to
+// AS and back is folded back to generic, and its loads become flat loads (and
+// constant structs land in vector registers instead of scalar ones).
+template <int AS, typename P> __device__ __forceinline__ P* uni_as(P* p) {
+  const unsigned long u = (unsigned long)uni((long)p);
+  return (P*)(__attribute__((address_space(AS))) P*)u;
+}
+template <typename P> __device__ __forceinline__ P* guni(P* p) { return uni_as<1>(p); }
+
+// The GPU hands consecutive blocks to its 8 XCDs in turn, and each XCD has
+// its own L2.  Block b of n runs on XCD b % 8; renumbering so that each XCD
+// takes a contiguous range of tiles keeps neighbouring tiles (the two halves
+// of a 128-byte line, for 64-byte runs) in one L2.
+__device__ __forceinline__ long xcd_remap(long b, long n) {
+  const long x = b & 7, i = b >> 3;
+  const long per = n >> 3, rem = n & 7;
+  return x * per + (x < rem ? x : rem) + i;
 }
 
 // Full transform of P pencils of length n.  Data starts in buf0; returns the
@@ -721,13 +736,25 @@ __device__ __forceinline__ void x_green_body(typename C2<T>::t* __restrict__ s, 
   const int nb = min(B, g.nzh - kz0);
   const int cnt = B * g.nx;
   C reg[5][E];
-  // component loops are unrolled so reg[][] stays in registers
+  // Every load of all 5 components is issued before the first transform, so
+  // a block keeps 5 B nx values in flight instead of waiting on one component
+  // at a time.  Component loops are unrolled so reg[][] stays in registers.
 #pragma unroll
   for (int c = 0; c < 5; c++) {
-    C* base = s + (long)c * g.nc + q0;
-    for (int e = threadIdx.x; e < cnt; e += blockDim.x) {
+    const C* base = s + (long)c * g.nc + q0;
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+      const int e = threadIdx.x + i * blockDim.x;
       const int b = e % B, x = e / B;
-      buf0[e] = b < nb ? base[(long)x * ncol + b] : cmake<T>(0, 0);
+      reg[c][i] = (e < cnt && b < nb) ? base[(long)x * ncol + b] : cmake<T>(0, 0);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 5; c++) {
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+      const int e = threadIdx.x + i * blockDim.x;
+      if (e < cnt) buf0[e] = reg[c][i];
     }
     __syncthreads();
     C* res = lds_fft<T, B>(buf0, buf1, ap, tw, -1);
@@ -776,7 +803,7 @@ __global__ void __launch_bounds__(NT) k_x_green(typename C2<T>::t* __restrict__ 
                                                  const typename C2<T>::t* __restrict__ tw,
                                                  const GreenConst<T>* __restrict__ gc, const int* __restrict__ done) {
   if (done && *done) return;
-  x_green_body<T, B, E>(s, g, ap, tw, gc, blockIdx.x);
+  x_green_body<T, B, E>(s, g, ap, tw, gc, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -823,7 +850,7 @@ __global__ void __launch_bounds__(256) k_z_r2c_b(FftGroup grp) {
   using C = typename C2<T>::t;
   long local;
   const FftJob* J = grp.jobs + group_member(grp, blockIdx.x, &local);
-  const int* done = uni(J->done);
+  const int* done = guni(J->done);
   if (done && *done) return;
   const FftGeom g = uni(J->g);
   const AxisPlan ap = uni_plan(J->az);
@@ -834,7 +861,7 @@ __global__ void __launch_bounds__(256) k_z_c2r_b(FftGroup grp) {
   using C = typename C2<T>::t;
   long local;
   const FftJob* J = grp.jobs + group_member(grp, blockIdx.x, &local);
-  const int* done = uni(J->done);
+  const int* done = guni(J->done);
   if (done && *done) return;
   const FftGeom g = uni(J->g);
   const AxisPlan ap = uni_plan(J->az);
@@ -846,7 +873,7 @@ __global__ void __launch_bounds__(256) k_y_b(FftGroup grp, int sign) {
   using C = typename C2<T>::t;
   long local;
   const FftJob* J = grp.jobs + group_member(grp, blockIdx.x, &local);
-  const int* done = uni(J->done);
+  const int* done = guni(J->done);
   if (done && *done) return;
   const FftGeom g = uni(J->g);
   const AxisPlan ap = uni_plan(J->ay);
@@ -856,12 +883,12 @@ template <typename T, int B, int E, int NT>
 __global__ void __launch_bounds__(NT) k_x_green_b(FftGroup grp) {
   using C = typename C2<T>::t;
   long local;
-  const FftJob* J = grp.jobs + group_member(grp, blockIdx.x, &local);
-  const int* done = uni(J->done);
+  const FftJob* J = grp.jobs + group_member(grp, xcd_remap(blockIdx.x, gridDim.x), &local);
+  const int* done = guni(J->done);
   if (done && *done) return;
   const FftGeom g = uni(J->g);
   const AxisPlan ap = uni_plan(J->ax);
-  x_green_body<T, B, E>(guni((C*)J->spec), g, ap, guni((const C*)J->tw[0]), uni((const GreenConst<T>*)J->gc), local);
+  x_green_body<T, B, E>(guni((C*)J->spec), g, ap, guni((const C*)J->tw[0]), uni_as<4>((const GreenConst<T>*)J->gc), local);
 }
 
 // ---------------------------------------------------------------------------

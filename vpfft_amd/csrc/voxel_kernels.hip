@@ -85,7 +85,8 @@ __device__ __forceinline__ void spd_inverse5(const double A[15], double Ainv[15]
 
 // ---------------------------------------------------------------------------
 // InitializeComputation + the secant moduli of BuildLinearRefMediumStiffness
-// for voxel v; adds M_loc^-1 (15) and the degenerate count (slot 15) to acc.
+// for voxel v; adds M_loc^-1 (15), the degenerate count (slot 15) and the
+// Taylor Newton steps (slot 16) to acc.
 template <typename T>
 __device__ __forceinline__ void prologue_voxel(long v, long n, const double* __restrict__ R,
                                                const double* __restrict__ crss, const StepConst& C,
@@ -111,7 +112,9 @@ __device__ __forceinline__ void prologue_voxel(long v, long n, const double* __r
     }
     if (!(mrss >= 1e-10)) acc[15] += 1.0;  // degenerate (the reference exits here)
     double s[5];
-    faithful::taylor_nr(guess, m, tau, C, 5e-4, 1000, s);
+    int tsteps = 0;
+    faithful::taylor_nr(guess, m, tau, C, 5e-4, 1000, s, &tsteps);
+    acc[16] += (double)tsteps;
 #pragma unroll
     for (int i = 0; i < 5; i++) {
       sigma[i * n + v] = s[i];
@@ -230,7 +233,7 @@ __device__ __forceinline__ void l0_finalize_body(long n, int nblocks, const doub
     derive_constants(C, g64, g32);
     for (int e = 0; e < 25; e++) rec->L0[e] = C->L0[e];
     rec->degenerate = sh[0][15];
-    rec->taylor_steps = 0.0;
+    rec->taylor_steps = sh[0][16];
   }
 }
 
@@ -278,29 +281,33 @@ __device__ __forceinline__ int vox_member(const VoxGroup& grp, long b, long* loc
   return vuni(grp.ids[lo]);
 }
 
-// generic pointers from a table, known to address global memory
-template <typename P> __device__ __forceinline__ P* as_global(P* p) {
-  return (P*)(__attribute__((address_space(1))) P*)p;
+// A uniform pointer from a job table, made directly in the global address
+// space from its integer value.  A generic pointer cast to global and back is
+// folded back to generic, and its loads become flat loads (and read-only
+// constant structs land in vector registers instead of scalar ones).
+template <typename P> __device__ __forceinline__ P* vuni_global(P* p) {
+  const unsigned long u = (unsigned long)vuni((long)p);
+  return (P*)(__attribute__((address_space(1))) P*)u;
 }
 
 // the job's fields in scalar registers
 __device__ __forceinline__ VoxJob vox_job(const VoxJob* J) {
   VoxJob r;
   r.n = vuni(J->n);
-  r.dtil = vuni(J->dtil);
-  r.phi = vuni(J->phi);
-  r.sigma = as_global(vuni(J->sigma));
-  r.lambda = as_global(vuni(J->lambda));
-  r.R = as_global(vuni(J->R));
-  r.crss = as_global(vuni(J->crss));
-  r.acc = as_global(vuni(J->acc));
-  r.C = vuni(J->C);
-  r.g64 = vuni(J->g64);
-  r.g32 = vuni(J->g32);
-  r.partial = as_global(vuni(J->partial));
-  r.ls = vuni(J->ls);
-  r.rec = vuni(J->rec);
-  r.prol = vuni(J->prol);
+  r.dtil = vuni_global(J->dtil);
+  r.phi = vuni_global(J->phi);
+  r.sigma = vuni_global(J->sigma);
+  r.lambda = vuni_global(J->lambda);
+  r.R = vuni_global(J->R);
+  r.crss = vuni_global(J->crss);
+  r.acc = vuni_global(J->acc);
+  r.C = vuni_global(J->C);
+  r.g64 = vuni_global(J->g64);
+  r.g32 = vuni_global(J->g32);
+  r.partial = vuni_global(J->partial);
+  r.ls = vuni_global(J->ls);
+  r.rec = vuni_global(J->rec);
+  r.prol = vuni_global(J->prol);
   return r;
 }
 
@@ -329,14 +336,14 @@ k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restr
     const VoxJob J = vox_job(grp.jobs + vox_member(grp, blockIdx.x, &blk));
     if (J.ls->done) return;
     n = J.n;
-    dtil = as_global((const T*)J.dtil);
-    phi = as_global((T*)J.phi);
-    sigma = as_global(J.sigma);
-    lambda = as_global(J.lambda);
-    R = as_global((const double*)J.R);
-    crss = as_global((const double*)J.crss);
+    dtil = (const T*)J.dtil;
+    phi = (T*)J.phi;
+    sigma = J.sigma;
+    lambda = J.lambda;
+    R = (const double*)J.R;
+    crss = (const double*)J.crss;
     Cp = J.C;
-    partial = as_global(J.partial);
+    partial = J.partial;
   } else {
     if (done && *done) return;
     n = n_;
