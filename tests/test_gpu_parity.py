@@ -107,7 +107,7 @@ def test_taylor_nr_batch_matches_oracle(gpu, vpfft, oracle, nexp):
 
 
 @pytest.mark.parametrize("dims", [(16, 16, 16), (8, 12, 10), (12, 10, 15), (24, 18, 20), (32, 16, 28), (4, 256, 10),
-                                  (6, 256, 256), (256, 4, 6)])
+                                  (6, 256, 256), (256, 4, 6), (256, 8, 36)])
 def test_strain_rate_evolution_matches_oracle(gpu, vpfft, oracle, dims):
     """FFT + Green + inverse FFT (R2C/C2R with the special-plane operator
     G_eff) against the reference's C2C formulation."""
@@ -122,6 +122,22 @@ def test_strain_rate_evolution_matches_oracle(gpu, vpfft, oracle, dims):
         d_gpu = g.StrainRateEvolutionWithFFT(L0, phi)
     d_ref = oracle.strain_rate_evolution(dims, mat, L0, phi, lengths=(1.0, 1.25, 0.8))
     assert rel_err(d_gpu, d_ref) < 1e-12, rel_err(d_gpu, d_ref)
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (256, 4, 6), (256, 8, 36), (6, 256, 256)])
+def test_strain_rate_evolution_mixed_precision(gpu, vpfft, oracle, dims):
+    """The FP32 transforms of the mixed form (256-point register passes and
+    the generic ones, ragged kz runs included) against the FP64 oracle."""
+    rng = np.random.default_rng(7 + sum(dims))
+    nx, ny, nz = dims
+    L0 = _random_spd(rng).T.reshape(25)
+    phi = rng.standard_normal((5, nx * ny * nz))
+    mat = vpfft.Material.uniform(1.0, 10)
+    with vpfft.MaterialGrid(nx, ny, nz, mat, precision=vpfft.MIXED) as g:
+        g.SetVoxelLength(1.0, 1.25, 0.8)
+        d_gpu = g.StrainRateEvolutionWithFFT(L0, phi)
+    d_ref = oracle.strain_rate_evolution(dims, mat, L0, phi, lengths=(1.0, 1.25, 0.8))
+    assert rel_err(d_gpu, d_ref) < 1e-5, rel_err(d_gpu, d_ref)
 
 
 def _run_pair(vpfft, oracle, dims, grains, nexp, edot, max_iter, dt, tol, seed, steps=1, precision=0):

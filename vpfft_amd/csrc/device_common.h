@@ -171,8 +171,20 @@ __device__ __forceinline__ void ldlt_solve5(double A[15], const double b[5], dou
   }
 }
 
+// 1/d within an ulp: the hardware reciprocal estimate refined by two Newton
+// steps (5 instructions against the 10 of an IEEE division).  For the Newton
+// kernel's own solve, where a last-bit difference only moves the iterate.
+__device__ __forceinline__ double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-d, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
 // In-place variant: A's packed lower triangle is overwritten by L (unit
-// diagonal implied), the solution replaces b.  Same arithmetic as ldlt_solve5.
+// diagonal implied), the solution replaces b.  The arithmetic of ldlt_solve5,
+// with the pivots' reciprocals from rcp_nr.
 __device__ __forceinline__ void ldlt_solve5_inplace(double A[15], double b[5]) {
   double dinv[5];
 #pragma unroll
@@ -183,7 +195,7 @@ __device__ __forceinline__ void ldlt_solve5_inplace(double A[15], double b[5]) {
     const double scale = fabs(A[sidx(j, j)]);
     const bool ok = d > 1e-300 && d > 1e-14 * scale;
     const double dj = ok ? d : 0.0;
-    dinv[j] = ok ? 1.0 / d : 0.0;
+    dinv[j] = ok ? rcp_nr(d) : 0.0;
 #pragma unroll
     for (int i = j + 1; i < 5; i++) {
       double t = A[sidx(i, j)];
@@ -326,7 +338,7 @@ __device__ __forceinline__ void inverse_map(const double R[9], double P[25]) {
   // R^-1 = adj(R) / det(R)
   const double c00 = R[4] * R[8] - R[5] * R[7], c01 = R[5] * R[6] - R[3] * R[8], c02 = R[3] * R[7] - R[4] * R[6];
   const double det = R[0] * c00 + R[1] * c01 + R[2] * c02;
-  const double id = 1.0 / det;
+  const double id = rcp_nr(det);
   double Ri[9];
   Ri[0] = c00 * id;
   Ri[3] = c01 * id;
@@ -345,7 +357,7 @@ __device__ __forceinline__ void inverse_map(const double R[9], double P[25]) {
   const double h00 = hcol(0, 0), h11 = hcol(1, 1), h22 = hcol(2, 2);
   proj_sym(h00, h11, h22, hcol(1, 2), hcol(0, 2), hcol(0, 1), h);
   proj_sym(krow(0, 0), krow(1, 1), krow(2, 2), krow(1, 2), krow(0, 2), krow(0, 1), k);
-  const double itr = 1.0 / (h00 + h11 + h22);
+  const double itr = rcp_nr(h00 + h11 + h22);
 #pragma unroll
   for (int a = 0; a < 5; a++) {
     const double ha = h[a] * itr;
@@ -467,21 +479,17 @@ __device__ __forceinline__ double max_rss(const double sc[5], const double inv_t
 
 // Solvers::SolveConstrainedConstitutiveEquations (Solvers.cpp:111-222) with
 // the Newton matrix built in the crystal frame.  cur: the sample-frame
-// iterate (in/out); R: the orientation (Q = Q(R) is rebuilt on a back-track);
+// iterate (in/out); cc: the same point in the crystal frame, Q cur (in;
+// overwritten); R: the orientation (Q = Q(R) is rebuilt on a back-track);
 // P = Q^-1; q = P^T (L0^-1 lambda + edot + d~); A0 = P^T L0^-1 P (packed
 // lower).  Budget, back-track and stopping rule as in the reference;
 // *err_out untouched when the budget runs out in the back-track.
 template <int NE>
-__device__ __forceinline__ void constrained_nr(double cur[5], const double R[9], const double P[25],
+__device__ __forceinline__ void constrained_nr(double cur[5], double cc[5], const double R[9], const double P[25],
                                                const double inv_tau[kSlips], const StepConst& C, const double q[5],
                                                const double A0[15], double eps, double max_rss, int max_it,
                                                double* err_out, int* steps_out) {
-  double cc[5], saved[5];
-  {
-    double Q[25];
-    dev_map(R, Q);
-    rot(Q, cur, cc);
-  }
+  double saved[5];
 #pragma unroll
   for (int i = 0; i < 5; i++) saved[i] = cur[i];
   int rem = max_it;

@@ -807,6 +807,139 @@ __global__ void __launch_bounds__(NT) k_x_green(typename C2<T>::t* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// x pass of length 256 in the 16 x 16 register form, Green operator fused.
+// Block = P consecutive kz columns of one y, all 5 components: component c
+// is threads [16 P c, 16 P (c + 1)), thread (c, t, p) owns x = t + 16 j of
+// pencil p (p fastest, so a load covers 16 rows of P consecutive kz).  The
+// five components transform side by side; the Green operator then needs all
+// five at one frequency, so the spectra go through LDS, in two halves of the
+// frequencies, to the threads that apply it, and back.
+// LDS: per component a 16 x 17 x P array for the FFT exchange (rows padded
+// to 17 so the transposing reads spread over the banks), reused for the
+// 2 x 5 x 128 x P reals of a half of the Green exchange.
+// ---------------------------------------------------------------------------
+template <int P> __device__ __forceinline__ int xex(int row, int col, int p) { return (row * 17 + col) * P + p; }
+
+// thread (t, p) holds V[k] (k = 0..15) of pencil p; afterwards it holds
+// W[k] = V_k[t] (as exchange16).  buf: this component's 16 x 17 x P reals.
+template <typename T, int P>
+__device__ __forceinline__ void exchange16_x(typename C2<T>::t v[16], T* buf, int t, int p) {
+  T re[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) buf[xex<P>(k, t, p)] = v[k].x;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) re[k] = buf[xex<P>(t, k, p)];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) buf[xex<P>(k, t, p)] = v[k].y;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) v[k] = cmake<T>(re[k], buf[xex<P>(t, k, p)]);
+  __syncthreads();
+}
+
+// 256-point transform of the pencil whose x = t + 16 j thread t holds in
+// v[j]; on return v[k2] holds frequency t + 16 k2 (natural order).
+template <typename T, int P>
+__device__ __forceinline__ void fft256_x(typename C2<T>::t v[16], T* buf, const typename C2<T>::t* __restrict__ tw,
+                                         int t, int p, int sign) {
+  using C = typename C2<T>::t;
+  dft16<T>(v, sign);
+#pragma unroll
+  for (int k = 1; k < 16; k++) {  // W256^(t k), sign as the transform
+    C w = tw[t * k];
+    if (sign > 0) w.y = -w.y;
+    v[k] = cmul(v[k], w);
+  }
+  exchange16_x<T, P>(v, buf, t, p);
+  dft16<T>(v, sign);
+}
+
+// Four waves per SIMD: three blocks per CU.  In FP64 that spills 52 B per
+// lane and still runs 14% faster than three waves (two blocks per CU)
+// without spills (256^3: 0.53 against 0.61 ms).
+template <typename T, int P>
+__global__ void __launch_bounds__(80 * P) __attribute__((amdgpu_waves_per_eu(4))) k_x256_green(typename C2<T>::t* __restrict__ s, FftGeom g,
+                                                        const typename C2<T>::t* __restrict__ tw,
+                                                        const GreenConst<T>* __restrict__ gc,
+                                                        const int* __restrict__ done) {
+  using C = typename C2<T>::t;
+  constexpr int NT = 80 * P;  // threads: 5 components x 16 P
+  if (done && *done) return;
+  __shared__ T buf[5 * 16 * 17 * P];
+  const int runs = (g.nzh + P - 1) / P;
+  const long blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int iy = (int)(blk / runs);
+  const int kz0 = (int)(blk - (long)iy * runs) * P;
+  const int tid = threadIdx.x;
+  const int c = __builtin_amdgcn_readfirstlane(tid / (16 * P));  // whole waves per component
+  const int r = tid - c * 16 * P;
+  const int p = r % P, t = r / P;
+  const bool active = kz0 + p < g.nzh;
+  // a uniform base and 32-bit element offsets (x <= 255 rows of ny nzp), so
+  // the addresses need one register each instead of a 64-bit pair
+  const unsigned ncol = (unsigned)(g.ny * g.nzp);  // x stride
+  C* base = s + (long)c * g.nc + (long)iy * g.nzp + kz0;
+  T* cbuf = buf + c * (16 * 17 * P);
+  C v[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) v[j] = active ? base[(unsigned)(t + 16 * j) * ncol + (unsigned)p] : cmake<T>(0, 0);
+  fft256_x<T, P>(v, cbuf, tw, t, p, -1);
+  // v[k2] = X_c(f = t + 16 k2) of pencil p.  The Green operator runs in two
+  // halves of the frequencies (f < 128 and f >= 128): each half of all five
+  // spectra, real and imaginary parts, fits in the same LDS, and the threads
+  // apply the operator in place one point at a time.  In half h the point
+  // q = (f - 128 h) P + pp holds component cc at buf[cc 128 P + q] (real)
+  // and buf[640 P + cc 128 P + q] (imaginary).
+  constexpr int NPT = (128 * P + NT - 1) / NT;  // Green points per thread per half
+  const GreenConst<T>* G = as_const_mem(gc);
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+#pragma unroll
+    for (int kk = 0; kk < 8; kk++) {
+      const int i = (c * 128 + t + 16 * kk) * P + p;
+      buf[i] = v[8 * h + kk].x;
+      buf[640 * P + i] = v[8 * h + kk].y;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int m = 0; m < NPT; m++) {
+      const int q = tid + NT * m;
+      const int iz = kz0 + q % P;
+      if (q < 128 * P && iz < g.nzh) {
+        C ph[5];
+#pragma unroll
+        for (int cc = 0; cc < 5; cc++) ph[cc] = cmake<T>(buf[cc * 128 * P + q], buf[640 * P + cc * 128 * P + q]);
+        green_apply<T>(ph, 128 * h + q / P, iy, iz, *G);
+#pragma unroll
+        for (int cc = 0; cc < 5; cc++) {
+          buf[cc * 128 * P + q] = ph[cc].x;
+          buf[640 * P + cc * 128 * P + q] = ph[cc].y;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 8; kk++) {
+      const int i = (c * 128 + t + 16 * kk) * P + p;
+      v[8 * h + kk] = cmake<T>(buf[i], buf[640 * P + i]);
+    }
+    __syncthreads();
+  }
+  // inverse: v[j] holds frequency t + 16 j, the layout the transform takes.
+  // The twiddles are loaded again (an opaque copy of the pointer) rather
+  // than held in registers from the forward transform.
+  const C* tw2 = tw;
+  __asm__ volatile("" : "+s"(tw2));
+  fft256_x<T, P>(v, cbuf, tw2, t, p, +1);
+  if (active) {
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) base[(unsigned)(t + 16 * k2) * ncol + (unsigned)p] = v[k2];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Batched passes: one launch covers the same pass of many transform jobs
 // (independent grids of any size).  Blocks [off[m], off[m+1]) of the launch
 // belong to job ids[m]; a job whose loop flag is set returns at entry.
@@ -995,6 +1128,13 @@ template <typename T>
 static hipError_t launch_x_green_t(void* s, const FftGeom& g, const AxisPlan& ap, const void* tw,
                                    const void* gc, int Bq, const int* done, hipStream_t st) {
   using C = typename C2<T>::t;
+  if (g.nx == 256 && Bq == 0) {  // the register 16 x 16 form: 64-byte runs of kz
+    constexpr int P = sizeof(T) == 8 ? 4 : 8;
+    const unsigned grid = (unsigned)((long)g.ny * ((g.nzh + P - 1) / P));
+    hipLaunchKernelGGL((k_x256_green<T, P>), dim3(grid), dim3(80 * P), 0, st, (C*)s, g, (const C*)tw,
+                       (const GreenConst<T>*)gc, done);
+    return hipGetLastError();
+  }
   const PassShape ps = pass_shape(sizeof(T) == 8 ? 0 : 1, kPassX, g, Bq);
   if (!ps.ok) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)ps.blocks;

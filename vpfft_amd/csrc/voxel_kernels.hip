@@ -379,16 +379,20 @@ k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restr
 #pragma unroll
     for (int i = 0; i < 5; i++) s[i] = sigma[i * n + v];
     // yield-surface rescale, MaterialGrid.cpp:954-959
-    double mrss;
+    double mrss, sc[5];
     {
-      double Q[25], sc[5];
+      double Q[25];
       cf::dev_map(r9, Q);
       cf::rot(Q, s, sc);
       mrss = cf::max_rss(sc, it);
     }
     if (!(mrss >= 1e-10)) acc[13] += 1.0;
+    const double imrss = rcp_nr(mrss);
 #pragma unroll
-    for (int i = 0; i < 5; i++) s[i] /= mrss;
+    for (int i = 0; i < 5; i++) {
+      s[i] /= mrss;
+      sc[i] *= imrss;  // the crystal-frame start point of the solve
+    }
     // q = L0^-1 lambda + edot + d~ ; lambda and d~ are re-read after the solve
     double q[5];
 #pragma unroll
@@ -406,7 +410,7 @@ k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restr
       cf::inverse_map(r9, P);
       cf::rot_t(P, q, qc);
       cf::congruence_t(P, C.L0inv, A0);
-      cf::constrained_nr<NE>(s, r9, P, it, C, qc, A0, 5e-4, 1e9, 100, &err, &steps);
+      cf::constrained_nr<NE>(s, sc, r9, P, it, C, qc, A0, 5e-4, 1e9, 100, &err, &steps);
     }
     double e[5];
     {
@@ -702,9 +706,12 @@ __global__ void k_constrained_batch(long n, const double* __restrict__ R, const 
     for (int i = 0; i < 5; i++) q[i] = Ll[i] + C.edot[i] + d[i];
     cf::rot_t(P, q, qc);
     cf::congruence_t(P, C.L0inv, A0);
+    double Q[25], sc[5];
+    cf::dev_map(r, Q);
+    cf::rot(Q, s, sc);
     double e = err[v];
     int st = 0;
-    cf::constrained_nr<0>(s, r, P, it, C, qc, A0, 5e-4, 1e9, 100, &e, &st);
+    cf::constrained_nr<0>(s, sc, r, P, it, C, qc, A0, 5e-4, 1e9, 100, &e, &st);
     for (int i = 0; i < 5; i++) s_out[5 * v + i] = s[i];
     err[v] = e;
     steps[v] = st;
