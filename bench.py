@@ -34,8 +34,8 @@ FP64_PEAK_TFLOPS = 78.6
 # congruence P^T L0^-1 P, the rescale, the constitutive law, the lambda update
 # and the block sums.  Per Newton step: 12 resolved shears, the power law, the
 # crystal-frame Jacobian and residual, the 5x5 LDL^T solve and the map back.
-NR_FLOPS_FIXED = 1900.0       # per voxel per iteration, outside the Newton steps
-NR_FLOPS_PER_STEP = 730.0     # per Newton step
+NR_FLOPS_FIXED = 1850.0       # per voxel per iteration, outside the Newton steps
+NR_FLOPS_PER_STEP = 570.0     # per Newton step
 # d~, sigma, lambda (40 B each), R (72 B) and one CRSS (8 B) read; sigma,
 # lambda, phi written (40 B each): the uniform-CRSS variant the workload runs
 # (every voxel's CRSS equal and kept equal by the hardening law)

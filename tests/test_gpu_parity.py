@@ -185,6 +185,23 @@ def test_strain_step_32cubed_matches_oracle(gpu, vpfft, oracle):
     assert abs(info.err_strain_rate - ref["err_strain_rate"]) <= 1e-6 * ref["err_strain_rate"]
 
 
+def test_strain_step_64cubed_512_grains_n20(gpu, vpfft, oracle):
+    """The parity configuration scaled to 64^3: 512 grains, n=20, tension,
+    eps 1e-6, at most 100 iterations; fields within 1e-8, same iteration
+    count."""
+    (avg, info, ref, f), = _run_pair(vpfft, oracle, (64, 64, 64), 512, 20, edot_tension(), 100, 0.005, 1e-6, 77)
+    _check_step(avg, info, ref, f)
+
+
+@pytest.mark.parametrize("dims", [(256, 256, 4), (4, 8, 256)])
+def test_strain_step_with_register_ffts(gpu, vpfft, oracle, dims):
+    """Whole steps on grids with 256-point axes, so the register-form passes
+    (x with the Green operator, y, and the z real/half-spectrum pair) run
+    inside the fixed-point loop."""
+    (avg, info, ref, f), = _run_pair(vpfft, oracle, dims, 32, 20, random_unit_edot(11), 100, 0.005, 1e-6, 21)
+    _check_step(avg, info, ref, f)
+
+
 def test_strain_step_converging_same_iteration_count(gpu, vpfft, oracle):
     """A loose tolerance so the loop exits on convergence, not on the cap."""
     (avg, info, ref, f), = _run_pair(vpfft, oracle, (24, 24, 24), 16, 10, random_unit_edot(3), 100, 0.005,

@@ -467,6 +467,54 @@ __device__ __forceinline__ double nexp(const StepConst& C, int k) {
   }
 }
 
+// Fast Walsh-Hadamard transform of 4 values: the sum and the three signed
+// sums (+,+,-,-), (+,-,-,+), (+,-,+,-).
+__device__ __forceinline__ void had4(double x0, double x1, double x2, double x3, double h[4]) {
+  const double p = x0 + x1, m = x0 - x1, q = x2 + x3, n = x2 - x3;
+  h[0] = p + q;
+  h[1] = p - q;
+  h[2] = m - n;
+  h[3] = m + n;
+}
+
+// S = sum_k kSign_k g_k and J = sum_k c_k kSign_k kSign_k^T (packed lower),
+// grouped by the structure of kSign: the slips k = f, f + 3, f + 6, f + 9 of
+// family f (= k mod 3) share their nonzero positions and differ only in
+// signs that are the rows of a 4 x 4 Hadamard matrix.  So each family
+// reduces to one 4-point Hadamard transform, and J takes ~35 additions
+// instead of 120.  (Same values as the direct sums up to rounding.)
+__device__ __forceinline__ void schmid_sums(const double g[kSlips], const double c[kSlips], double S[5],
+                                            double J[15]) {
+  double A[4], B[4], Cc[4], Ga[4], Gb[4], Gc[4];
+  had4(c[0], c[3], c[6], c[9], A);
+  had4(c[1], c[4], c[7], c[10], B);
+  had4(c[2], c[5], c[8], c[11], Cc);
+  had4(g[0], g[3], g[6], g[9], Ga);
+  had4(g[1], g[4], g[7], g[10], Gb);
+  had4(g[2], g[5], g[8], g[11], Gc);
+  const double ab = A[0] + B[0];
+  J[sidx(0, 0)] = __builtin_fma(4.0, Cc[0], ab);
+  J[sidx(1, 0)] = B[0] - A[0];
+  J[sidx(1, 1)] = ab;
+  J[sidx(2, 0)] = __builtin_fma(-2.0, Cc[3], -B[3]);
+  J[sidx(2, 1)] = -B[3];
+  J[sidx(2, 2)] = B[0] + Cc[0];
+  J[sidx(3, 0)] = __builtin_fma(2.0, Cc[1], A[1]);
+  J[sidx(3, 1)] = -A[1];
+  J[sidx(3, 2)] = -Cc[2];
+  J[sidx(3, 3)] = A[0] + Cc[0];
+  J[sidx(4, 0)] = A[2] - B[2];
+  J[sidx(4, 1)] = -A[2] - B[2];
+  J[sidx(4, 2)] = B[1];
+  J[sidx(4, 3)] = A[3];
+  J[sidx(4, 4)] = ab;
+  S[0] = __builtin_fma(-2.0, Gc[0], Ga[2] - Gb[0]);
+  S[1] = -Ga[2] - Gb[0];
+  S[2] = Gb[3] + Gc[3];
+  S[3] = Ga[3] - Gc[1];
+  S[4] = Ga[0] + Gb[2];
+}
+
 // max_k |m0_k . sc| / tau_k (MaterialGrid.h:550-566)
 __device__ __forceinline__ double max_rss(const double sc[5], const double inv_tau[kSlips]) {
   double u[5];
@@ -524,35 +572,14 @@ __device__ __forceinline__ void constrained_nr(double cur[5], double cc[5], cons
     } while (!ok);
     steps++;
     // S_c = sum_k m0_k gdot_k r_k ; J_c = sum_k n gdot_k / tau_k m0_k m0_k^T
-    double S[5], J[15];
-#pragma unroll
-    for (int i = 0; i < 5; i++) S[i] = -0.0;
-#pragma unroll
-    for (int e = 0; e < 15; e++) J[e] = -0.0;
+    double gr[kSlips], ck[kSlips], S[5], J[15];
 #pragma unroll
     for (int k = 0; k < kSlips; k++) {
       const double g = gamma_nm1<NE>(C, k, fabs(r[k]));
-      const double gr = g * r[k];
-      const double ck = nexp<NE>(C, k) * g * inv_tau[k];
-#pragma unroll
-      for (int i = 0; i < 5; i++) {
-        const int si = kSign[k][i];
-        if (si == 0) continue;
-        S[i] += (double)si * gr;
-#pragma unroll
-        for (int j = 0; j <= i; j++) {
-          const int sij = si * kSign[k][j];  // 0, +-1, +-2 or 4
-          if (sij == 0) continue;
-          if (sij == 1) {
-            J[sidx(i, j)] += ck;
-          } else if (sij == -1) {
-            J[sidx(i, j)] -= ck;
-          } else {
-            J[sidx(i, j)] += (double)sij * ck;
-          }
-        }
-      }
+      gr[k] = g * r[k];
+      ck[k] = nexp<NE>(C, k) * g * inv_tau[k];
     }
+    schmid_sums(gr, ck, S, J);
     // rhs_c = q - A0 cc - S_c ;  (A0 + J_c) d_c = rhs_c ; d = P d_c
     double rhs[5];
 #pragma unroll
