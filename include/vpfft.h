@@ -57,7 +57,9 @@ enum {
   VPFFT_FIELD_POLARIZATION = 4,  /* 5 x N  real part of _PolarizationField after the last iteration */
   VPFFT_FIELD_STRAIN_RATE_VARIATION = 5, /* 5 x N  d~ of the last iteration (DisplacementVariation real part) */
   VPFFT_FIELD_ACC_SHEAR = 6,     /* N      (AccumulatedShear, MaterialGrid.h:293) */
-  VPFFT_FIELD_SCHMID = 7         /* N x 60 (LocalSchmidtTensor, MaterialGrid.h:292), download only */
+  VPFFT_FIELD_SCHMID = 7,        /* N x 60 (LocalSchmidtTensor, MaterialGrid.h:292), download only */
+  VPFFT_FIELD_STRAIN_RATE = 8    /* N x 5  Solvers::ApplyConstitutiveEquations (Solvers.cpp:255-274) of the
+                                    current stress, orientation and CRSS at every voxel, download only */
 };
 
 typedef struct {

@@ -202,6 +202,22 @@ def test_strain_step_with_register_ffts(gpu, vpfft, oracle, dims):
     _check_step(avg, info, ref, f)
 
 
+def test_strain_rate_field_download(gpu, vpfft, oracle):
+    """VPFFT_FIELD_STRAIN_RATE: the constitutive law at every voxel of the
+    converged stress, against the oracle's ApplyConstitutiveEquations."""
+    dims = (12, 10, 8)
+    R, _ = polycrystal(None, 6, 31, dims)
+    mat = vpfft.Material.uniform(1.0, 10)
+    with vpfft.MaterialGrid(*dims, mat) as g:
+        g.SetVoxelLength(1.0, 1.0, 1.0)
+        g.SendDataAndInitialize(R)
+        g.RunSingleStrainStep(edot_tension(), 20, 0.005, 1e-5)
+        sig, Rn, crss, m = g.Stress(), g.Orientation(), g.CRSS(), g.SchmidtTensorList()
+        e = g.StrainRate()
+    ref = np.stack([oracle.apply_constitutive(sig[v], m[v], crss[v], mat) for v in range(sig.shape[0])])
+    assert rel_err(e, ref) < 1e-12, rel_err(e, ref)
+
+
 def test_strain_step_converging_same_iteration_count(gpu, vpfft, oracle):
     """A loose tolerance so the loop exits on convergence, not on the cap."""
     (avg, info, ref, f), = _run_pair(vpfft, oracle, (24, 24, 24), 16, 10, random_unit_edot(3), 100, 0.005,

@@ -32,6 +32,7 @@ FIELD_POLARIZATION = 4
 FIELD_STRAIN_RATE_VARIATION = 5
 FIELD_ACC_SHEAR = 6
 FIELD_SCHMID = 7
+FIELD_STRAIN_RATE = 8
 
 ERRORS = {
     -1: "VPFFT_E_INVALID",
@@ -420,6 +421,11 @@ class MaterialGrid:
 
     def SchmidtTensorList(self) -> np.ndarray:
         return self._download(FIELD_SCHMID, (self.N, 12, 5))
+
+    def StrainRate(self) -> np.ndarray:
+        """Solvers::ApplyConstitutiveEquations (Solvers.cpp:255-274) of the
+        current stress at every voxel, N x 5."""
+        return self._download(FIELD_STRAIN_RATE, (self.N, 5))
 
     def PolarizationField(self) -> np.ndarray:
         return self._download(FIELD_POLARIZATION, (5, self.N))

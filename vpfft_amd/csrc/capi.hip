@@ -56,6 +56,33 @@ __global__ void k_schmid_aos(const double* __restrict__ R, long n, long offset, 
   }
 }
 
+// the material constants the constitutive law reads, passed by value
+struct RateLaw {
+  double gdot0[kSlips];
+  int nexp[kSlips];
+};
+
+// Solvers::ApplyConstitutiveEquations (Solvers.cpp:255-274) at every voxel of
+// [offset, offset + count): strain rate from the stress, orientation and
+// CRSS fields, AoS (5 per voxel) into out.
+__global__ void k_strain_rate_aos(const double* __restrict__ R, const double* __restrict__ crss,
+                                  const double* __restrict__ sigma, long n, long offset, long count, RateLaw law,
+                                  double* __restrict__ out) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long)gridDim.x * blockDim.x) {
+    const long v = offset + t;
+    double r[9], m[kSlips][5], s[5], e[5] = {0, 0, 0, 0, 0};
+    for (int q = 0; q < 9; q++) r[q] = R[q * n + v];
+    for (int i = 0; i < 5; i++) s[i] = sigma[i * n + v];
+    schmid_from_R(r, m);
+    for (int k = 0; k < kSlips; k++) {
+      const double rss = dot5(m[k], s) / crss[k * n + v];
+      const double g = law.nexp[k] > 0 ? law.gdot0[k] * ipow(fabs(rss), law.nexp[k]) : law.gdot0[k];
+      for (int i = 0; i < 5; i++) e[i] += m[k][i] * g * sgn(rss);
+    }
+    for (int i = 0; i < 5; i++) out[5 * t + i] = e[i];
+  }
+}
+
 static int fill_blocks(long n) {
   long b = (n + 255) / 256;
   return (int)std::max(1L, std::min(b, 8192L));
@@ -1262,6 +1289,24 @@ int vpfft_grid_download(void* grid, int field, double* host) {
                                       g->st);
         if (e == hipSuccess) e = hipStreamSynchronize(g->st);
         if (e != hipSuccess) return set_error(VPFFT_E_DEVICE, "download schmid: %s", hipGetErrorString(e));
+      }
+      return VPFFT_OK;
+    }
+    case VPFFT_FIELD_STRAIN_RATE: {
+      vpfft::RateLaw law;
+      for (int k = 0; k < vpfft::kSlips; k++) {
+        law.gdot0[k] = g->mat.gamma_dot_base[k];
+        law.nexp[k] = g->mat.rate_sensitivity[k];
+      }
+      const long chunk = g->staging_elems / 5;
+      for (long off = 0; off < g->n; off += chunk) {
+        const long cnt = std::min(chunk, g->n - off);
+        hipLaunchKernelGGL(vpfft::k_strain_rate_aos, dim3(vpfft::fill_blocks(cnt)), dim3(256), 0, g->st, g->R,
+                           g->crss, g->sigma, g->n, off, cnt, law, g->staging);
+        hipError_t e = hipMemcpyAsync(host + 5 * off, g->staging, sizeof(double) * 5 * cnt, hipMemcpyDeviceToHost,
+                                      g->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(g->st);
+        if (e != hipSuccess) return set_error(VPFFT_E_DEVICE, "download strain rate: %s", hipGetErrorString(e));
       }
       return VPFFT_OK;
     }

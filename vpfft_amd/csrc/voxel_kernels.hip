@@ -83,6 +83,24 @@ __device__ __forceinline__ void spd_inverse5(const double A[15], double Ainv[15]
   }
 }
 
+// Voxel of thread tid in block blk of a one-voxel-per-thread launch.  When Ny % 4 == 0 and Nz % 16 == 0
+// each wave takes a 4 (y) x 16 (z) tile of one x-plane instead of 64
+// consecutive z: 16-lane runs still read whole 128-byte lines, and a compact
+// tile spans fewer grains (256^3 with 4096 Voronoi grains: 2.9 on average
+// against 6.6 for a 64-voxel line), so its lanes' Newton step counts (Taylor in
+// the prologue, constrained in the loop) differ less and the wave idles less
+// on its slowest lane.  Otherwise a block takes
+// 256 consecutive voxels.  Threads past the grid get v >= N.
+__device__ __forceinline__ long tile_voxel(long blk, int tid, int ny, int nz) {
+  if ((ny & 3) || (nz & 15)) return blk * kPointThreads + tid;
+  const long tile = blk * (kPointThreads / 64) + __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const long ztn = nz >> 4, ytn = ny >> 2;
+  const long zt = tile % ztn, rest = tile / ztn;
+  const long yt = rest % ytn, x = rest / ytn;
+  return (x * ny + yt * 4 + (lane >> 4)) * nz + zt * 16 + (lane & 15);
+}
+
 // ---------------------------------------------------------------------------
 // InitializeComputation + the secant moduli of BuildLinearRefMediumStiffness
 // for voxel v; adds M_loc^-1 (15), the degenerate count (slot 15) and the
@@ -152,8 +170,12 @@ __global__ void __launch_bounds__(kPointThreads) k_prologue(long n, const double
 #pragma unroll
   for (int w = 0; w < 17; w++) acc[w] = 0.0;
   double mx[1] = {0.0};
-  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x)
-    prologue_voxel<T>(v, n, R, crss, C, field, sigma, lambda, acc);
+  // linear index i walks the blocks of tile_voxel (a permutation of the voxels)
+  for (long i = (long)blockIdx.x * kPointThreads + threadIdx.x; i < (long)nr_blocks(n) * kPointThreads;
+       i += (long)gridDim.x * kPointThreads) {
+    const long v = tile_voxel(i / kPointThreads, threadIdx.x, C.ny, C.nz);
+    if (v < n) prologue_voxel<T>(v, n, R, crss, C, field, sigma, lambda, acc);
+  }
   block_sum<17>(acc, mx, partial + (long)blockIdx.x * 18);
 }
 
@@ -311,23 +333,6 @@ __device__ __forceinline__ VoxJob vox_job(const VoxJob* J) {
   return r;
 }
 
-// Voxel of thread tid in Newton block blk.  When Ny % 4 == 0 and Nz % 16 == 0
-// each wave takes a 4 (y) x 16 (z) tile of one x-plane instead of 64
-// consecutive z: 16-lane runs still read whole 128-byte lines, and a compact
-// tile spans fewer grains (256^3 with 4096 Voronoi grains: 2.9 on average
-// against 6.6 for a 64-voxel line), so its lanes' Newton step counts differ
-// less and the wave idles less on its slowest lane.  Otherwise a block takes
-// 256 consecutive voxels.  Threads past the grid get v >= N.
-__device__ __forceinline__ long nr_voxel(long blk, int tid, int ny, int nz) {
-  if ((ny & 3) || (nz & 15)) return blk * kPointThreads + tid;
-  const long tile = blk * (kPointThreads / 64) + __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int lane = tid & 63;
-  const long ztn = nz >> 4, ytn = ny >> 2;
-  const long zt = tile % ztn, rest = tile / ztn;
-  const long yt = rest % ytn, x = rest / ytn;
-  return (x * ny + yt * 4 + (lane >> 4)) * nz + zt * 16 + (lane & 15);
-}
-
 // ---------------------------------------------------------------------------
 // K4: per-voxel body of the fixed-point loop (MaterialGrid.cpp:946-1003),
 // with the Newton solve in the crystal frame of the voxel.
@@ -380,7 +385,7 @@ k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restr
   for (int w = 0; w < 15; w++) acc[w] = 0.0;
   double mx[2] = {0.0, 0.0};  // max NR error, max NR steps
   // one voxel per lane: nothing but the voxel's own state is live
-  const long v = nr_voxel(blk, threadIdx.x, C.ny, C.nz);
+  const long v = tile_voxel(blk, threadIdx.x, C.ny, C.nz);
   if (v < n) {
     double it[kSlips], r9[9];
     load_R(R, n, v, r9);
@@ -647,8 +652,9 @@ __global__ void __launch_bounds__(kPointThreads) k_prologue_b(VoxGroup grp) {
 #pragma unroll
   for (int w = 0; w < 17; w++) acc[w] = 0.0;
   double mx[1] = {0.0};
-  const long v = local * kPointThreads + threadIdx.x;
-  if (v < J.n) prologue_voxel<T>(v, J.n, J.R, J.crss, *as_const_mem(J.C), (T*)J.phi, J.sigma, J.lambda, acc);
+  const StepConst& C = *as_const_mem(J.C);
+  const long v = tile_voxel(local, threadIdx.x, C.ny, C.nz);
+  if (v < J.n) prologue_voxel<T>(v, J.n, J.R, J.crss, C, (T*)J.phi, J.sigma, J.lambda, acc);
   block_sum<17>(acc, mx, J.partial + local * 18);
 }
 
