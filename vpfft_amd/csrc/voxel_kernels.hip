@@ -101,6 +101,20 @@ __device__ __forceinline__ long tile_voxel(long blk, int tid, int ny, int nz) {
   return (x * ny + yt * 4 + (lane >> 4)) * nz + zt * 16 + (lane & 15);
 }
 
+// The prologue's mapping: a wave takes a 4 x 4 x 4 cube when every edge is a
+// multiple of 4 (1.96 grains per wave on average at 256^3 / 4096 grains),
+// else the tile_voxel mapping.  Its loads are 32-byte runs, which the
+// prologue, bound by its Taylor solves and not by HBM, can afford.
+__device__ __forceinline__ long cube_voxel(long blk, int tid, int nx, int ny, int nz) {
+  if ((nx & 3) || (ny & 3) || (nz & 3)) return tile_voxel(blk, tid, ny, nz);
+  const long tile = blk * (kPointThreads / 64) + __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const long ztn = nz >> 2, ytn = ny >> 2;
+  const long zt = tile % ztn, rest = tile / ztn;
+  const long yt = rest % ytn, xt = rest / ytn;
+  return ((xt * 4 + (lane >> 4)) * ny + yt * 4 + ((lane >> 2) & 3)) * nz + zt * 4 + (lane & 3);
+}
+
 // ---------------------------------------------------------------------------
 // InitializeComputation + the secant moduli of BuildLinearRefMediumStiffness
 // for voxel v; adds M_loc^-1 (15), the degenerate count (slot 15) and the
@@ -170,10 +184,10 @@ __global__ void __launch_bounds__(kPointThreads) k_prologue(long n, const double
 #pragma unroll
   for (int w = 0; w < 17; w++) acc[w] = 0.0;
   double mx[1] = {0.0};
-  // linear index i walks the blocks of tile_voxel (a permutation of the voxels)
+  // linear index i walks the blocks of cube_voxel (a permutation of the voxels)
   for (long i = (long)blockIdx.x * kPointThreads + threadIdx.x; i < (long)nr_blocks(n) * kPointThreads;
        i += (long)gridDim.x * kPointThreads) {
-    const long v = tile_voxel(i / kPointThreads, threadIdx.x, C.ny, C.nz);
+    const long v = cube_voxel(i / kPointThreads, threadIdx.x, C.nx, C.ny, C.nz);
     if (v < n) prologue_voxel<T>(v, n, R, crss, C, field, sigma, lambda, acc);
   }
   block_sum<17>(acc, mx, partial + (long)blockIdx.x * 18);
@@ -653,7 +667,7 @@ __global__ void __launch_bounds__(kPointThreads) k_prologue_b(VoxGroup grp) {
   for (int w = 0; w < 17; w++) acc[w] = 0.0;
   double mx[1] = {0.0};
   const StepConst& C = *as_const_mem(J.C);
-  const long v = tile_voxel(local, threadIdx.x, C.ny, C.nz);
+  const long v = cube_voxel(local, threadIdx.x, C.nx, C.ny, C.nz);
   if (v < J.n) prologue_voxel<T>(v, J.n, J.R, J.crss, C, (T*)J.phi, J.sigma, J.lambda, acc);
   block_sum<17>(acc, mx, J.partial + local * 18);
 }
