@@ -806,7 +806,8 @@ static void fill_vox_job(Grid* g, LoopState* ls, VoxJob* J) {
 }
 
 static long pass_blocks(const Grid* g, int kind, int B) {
-  if (kind == L_PRO || kind == L_NR || kind == L_EPI) return nr_blocks(g->n);
+  if (kind == L_NR) return nr_blocks(g->n);
+  if (kind == L_PRO || kind == L_EPI) return pt_blocks(g->n);
   const FftPass pass = (kind == L_ZF || kind == L_ZI) ? kPassZ : ((kind == L_YF || kind == L_YI) ? kPassY : kPassX);
   return pass_shape(g->prec, pass, g->geom, B).blocks;
 }
@@ -980,7 +981,7 @@ static int batch_step(Batch* b, Grid** gs, int count, const double* edot, int ma
       b->hoff[off_at] = 0;
       for (size_t m = 0; m < kv.second.size(); m++) {
         b->hids[ids_at + m] = kv.second[m];
-        L.blocks += nr_blocks(gs[kv.second[m]]->n);
+        L.blocks += pt_blocks(gs[kv.second[m]]->n);
         b->hoff[off_at + m + 1] = L.blocks;
       }
       ids_at += L.count;

@@ -13,8 +13,13 @@ constexpr int kPointThreads = 256;
 constexpr int kNrWaves = 2;        // waves per SIMD the Newton kernel is compiled for
 constexpr int kPartialStride = 24;  // 15 sums + 2 maxima per block, padded
 
-// the Newton kernel runs one voxel per lane
-__host__ __device__ inline int nr_blocks(long n) { return (int)((n + kPointThreads - 1) / kPointThreads); }
+// The Newton kernel runs one voxel per lane in one-wave blocks: no barrier,
+// so a wave retires when its own lanes are done instead of waiting for the
+// slowest wave of a larger block.
+constexpr int kNrThreads = 64;
+__host__ __device__ inline int nr_blocks(long n) { return (int)((n + kNrThreads - 1) / kNrThreads); }
+// blocks of a one-voxel-per-thread launch of kPointThreads-thread blocks
+__host__ __device__ inline int pt_blocks(long n) { return (int)((n + kPointThreads - 1) / kPointThreads); }
 // partial-sum buffer rows needed by launch_nr (+ its folded rows)
 inline long nr_partial_rows(long n) { return nr_blocks(n) + (nr_blocks(n) + 255) / 256; }
 

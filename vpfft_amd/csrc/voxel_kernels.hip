@@ -59,6 +59,25 @@ __device__ __forceinline__ void block_sum(double (&v)[W], double (&vmax)[M], dou
   }
 }
 
+// Reduce W sums and M maxima over one wave in a fixed order; lane 0 writes
+// them to out[0..W+M).
+template <int W, int M>
+__device__ __forceinline__ void wave_sum(double (&v)[W], double (&vmax)[M], double* out) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int w = 0; w < W; w++) v[w] += __shfl_xor(v[w], off);
+#pragma unroll
+    for (int w = 0; w < M; w++) vmax[w] = fmax(vmax[w], __shfl_xor(vmax[w], off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int w = 0; w < W; w++) out[w] = v[w];
+#pragma unroll
+    for (int w = 0; w < M; w++) out[W + w] = vmax[w];
+  }
+}
+
 __device__ __forceinline__ void load_R(const double* __restrict__ R, long n, long v, double r[9]) {
 #pragma unroll
   for (int e = 0; e < 9; e++) r[e] = R[e * n + v];
@@ -90,10 +109,11 @@ __device__ __forceinline__ void spd_inverse5(const double A[15], double Ainv[15]
 // against 6.6 for a 64-voxel line), so its lanes' Newton step counts (Taylor in
 // the prologue, constrained in the loop) differ less and the wave idles less
 // on its slowest lane.  Otherwise a block takes
-// 256 consecutive voxels.  Threads past the grid get v >= N.
+// THREADS consecutive voxels.  Threads past the grid get v >= N.
+template <int THREADS>
 __device__ __forceinline__ long tile_voxel(long blk, int tid, int ny, int nz) {
-  if ((ny & 3) || (nz & 15)) return blk * kPointThreads + tid;
-  const long tile = blk * (kPointThreads / 64) + __builtin_amdgcn_readfirstlane(tid >> 6);
+  if ((ny & 3) || (nz & 15)) return blk * THREADS + tid;
+  const long tile = blk * (THREADS / 64) + __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
   const long ztn = nz >> 4, ytn = ny >> 2;
   const long zt = tile % ztn, rest = tile / ztn;
@@ -106,7 +126,7 @@ __device__ __forceinline__ long tile_voxel(long blk, int tid, int ny, int nz) {
 // else the tile_voxel mapping.  Its loads are 32-byte runs, which the
 // prologue, bound by its Taylor solves and not by HBM, can afford.
 __device__ __forceinline__ long cube_voxel(long blk, int tid, int nx, int ny, int nz) {
-  if ((nx & 3) || (ny & 3) || (nz & 3)) return tile_voxel(blk, tid, ny, nz);
+  if ((nx & 3) || (ny & 3) || (nz & 3)) return tile_voxel<kPointThreads>(blk, tid, ny, nz);
   const long tile = blk * (kPointThreads / 64) + __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
   const long ztn = nz >> 2, ytn = ny >> 2;
@@ -185,7 +205,7 @@ __global__ void __launch_bounds__(kPointThreads) k_prologue(long n, const double
   for (int w = 0; w < 17; w++) acc[w] = 0.0;
   double mx[1] = {0.0};
   // linear index i walks the blocks of cube_voxel (a permutation of the voxels)
-  for (long i = (long)blockIdx.x * kPointThreads + threadIdx.x; i < (long)nr_blocks(n) * kPointThreads;
+  for (long i = (long)blockIdx.x * kPointThreads + threadIdx.x; i < (long)pt_blocks(n) * kPointThreads;
        i += (long)gridDim.x * kPointThreads) {
     const long v = cube_voxel(i / kPointThreads, threadIdx.x, C.nx, C.ny, C.nz);
     if (v < n) prologue_voxel<T>(v, n, R, crss, C, field, sigma, lambda, acc);
@@ -358,7 +378,7 @@ __device__ __forceinline__ VoxJob vox_job(const VoxJob* J) {
 // BATCHED: the grid's pointers come from the job table of grp (blocks of
 // many grids in one launch); else from the kernel arguments.
 template <typename T, int NE, bool UNIFORM_TAU, bool BATCHED>
-__global__ void __launch_bounds__(kPointThreads) __attribute__((amdgpu_waves_per_eu(kNrWaves, kNrWaves)))
+__global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu(kNrWaves, kNrWaves)))
 k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restrict__ sigma_,
      double* __restrict__ lambda_, const double* __restrict__ R_, const double* __restrict__ crss_,
      const StepConst* __restrict__ Cp_, double* __restrict__ partial_, const int* __restrict__ done, VoxGroup grp) {
@@ -399,7 +419,7 @@ k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restr
   for (int w = 0; w < 15; w++) acc[w] = 0.0;
   double mx[2] = {0.0, 0.0};  // max NR error, max NR steps
   // one voxel per lane: nothing but the voxel's own state is live
-  const long v = tile_voxel(blk, threadIdx.x, C.ny, C.nz);
+  const long v = tile_voxel<kNrThreads>(blk, threadIdx.x, C.ny, C.nz);
   if (v < n) {
     double it[kSlips], r9[9];
     load_R(R, n, v, r9);
@@ -495,7 +515,7 @@ k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restr
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
   if ((threadIdx.x & 63) == 0) acc[14] = 64.0 * wmax;
-  block_sum<15>(acc, mx, partial + blk * kPartialStride);
+  wave_sum<15>(acc, mx, partial + blk * kPartialStride);
 }
 
 
@@ -674,7 +694,7 @@ __global__ void __launch_bounds__(kPointThreads) k_prologue_b(VoxGroup grp) {
 
 __global__ void __launch_bounds__(256) k_L0_finalize_b(VoxGroup grp) {
   const VoxJob J = vox_job(grp.jobs + vuni(grp.ids[blockIdx.x]));
-  l0_finalize_body(J.n, nr_blocks(J.n), J.partial, J.C, J.g64, J.g32, J.prol);
+  l0_finalize_body(J.n, pt_blocks(J.n), J.partial, J.C, J.g64, J.g32, J.prol);
 }
 
 __global__ void k_loop_reset_b(VoxGroup grp) {
@@ -799,7 +819,7 @@ hipError_t launch_nr(int prec, int nexp_uniform, bool uniform_tau, long n, const
                      const int* done, hipStream_t st) {
   const int nb = nr_blocks(n);
 #define NR_LAUNCH(TT, NN, UU)                                                                                    \
-  hipLaunchKernelGGL((k_nr<TT, NN, UU, false>), dim3(nb), dim3(kPointThreads), 0, st, n, (const TT*)dtil, (TT*)phi, \
+  hipLaunchKernelGGL((k_nr<TT, NN, UU, false>), dim3(nb), dim3(kNrThreads), 0, st, n, (const TT*)dtil, (TT*)phi, \
                      sigma, lambda, R, crss, C, partial, done, VoxGroup{})
 #define NR_N(TT, UU)                                 \
   switch (nexp_uniform) {                             \
@@ -903,7 +923,7 @@ hipError_t launch_loop_reset_group(const VoxGroup& grp, hipStream_t st) {
 hipError_t launch_nr_group(int prec, int nexp_uniform, bool uniform_tau, const VoxGroup& grp, long blocks,
                            hipStream_t st) {
 #define NR_LAUNCH(TT, NN, UU)                                                                                   \
-  hipLaunchKernelGGL((k_nr<TT, NN, UU, true>), dim3((unsigned)blocks), dim3(kPointThreads), 0, st, 0L,          \
+  hipLaunchKernelGGL((k_nr<TT, NN, UU, true>), dim3((unsigned)blocks), dim3(kNrThreads), 0, st, 0L,             \
                      (const TT*)nullptr, (TT*)nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, grp)
 #define NR_N(TT, UU)                                 \
   switch (nexp_uniform) {                             \
