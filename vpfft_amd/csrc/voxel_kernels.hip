@@ -524,9 +524,12 @@ k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restr
 __global__ void __launch_bounds__(256) k_fold_partials(const double* __restrict__ in, int rows,
                                                        double* __restrict__ out, const int* __restrict__ done) {
   if (done && *done) return;
-  __shared__ double sh[256][17];
-  const long r = (long)blockIdx.x * 256 + threadIdx.x;
-  for (int w = 0; w < 17; w++) sh[threadIdx.x][w] = r < rows ? in[r * kPartialStride + w] : 0.0;
+  // the block's 256 rows are one contiguous chunk: load it coalesced
+  __shared__ double sh[256][kPartialStride];
+  double* flat = &sh[0][0];
+  const long r0 = (long)blockIdx.x * 256;
+  const long valid = (rows - r0 < 256 ? rows - r0 : 256) * kPartialStride;
+  for (int e = threadIdx.x; e < 256 * kPartialStride; e += 256) flat[e] = e < valid ? in[r0 * kPartialStride + e] : 0.0;
   __syncthreads();
   for (int stride = 128; stride > 0; stride >>= 1) {
     if (threadIdx.x < stride) {
