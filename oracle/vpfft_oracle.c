@@ -21,6 +21,15 @@
  *     ill-conditioned Taylor initialisation bit for bit.
  * These change results only at rounding level (~1e-16 relative).
  *
+ * Parity unpinned: nothing can reproduce the reference binary here (Eigen
+ * and FFTW-MPI are absent and the reference ships no fixtures), so
+ * vo_strain_step's iteration count and per-voxel Newton trajectories on a
+ * heterogeneous polycrystal are pinned only by the consistency of its own
+ * pieces (each checked against closed forms, invariants and numpy) and by
+ * its special cases (homogeneous single crystal, iteration cap, degenerate
+ * loading); the exact roundings of Eigen's LU / inverse and of FFTW are
+ * unpinned for the same reason.
+ *
  * Per-voxel work runs in parallel with OpenMP, but every reduction the
  * reference performs in voxel order (running mean of L0, the per-iteration
  * sums) is done sequentially in that same order, so results do not depend on
