@@ -326,11 +326,18 @@ def main():
         mx = run_form(V, R, mat, edot, args, V.MIXED, args.steps, args.warmup, profile=False, field=True)
         v_mx = mx["vox_iters"] / mx["wall"]
         err = float(np.linalg.norm(mx["avgs"][0] - f64["avgs"][0]) / np.linalg.norm(f64["avgs"][0]))
-        # per-voxel: max over voxels and components, relative to the field's max magnitude
-        ferr = float(np.abs(mx["sigma1"] - f64["sigma1"]).max() / np.abs(f64["sigma1"]).max())
+        # per-voxel: each voxel's largest component difference relative to the
+        # field's max magnitude; its maximum, median, 99.99th percentile and
+        # the count above 1e-3 (at n=20 a few voxels of the FP64 form itself
+        # move by ~1e-3 under a 1e-12 relative change of the strain rate)
+        dv = np.abs(mx["sigma1"] - f64["sigma1"]).max(axis=1) / np.abs(f64["sigma1"]).max()
         del mx["sigma1"], f64["sigma1"]
+        ferr = float(dv.max())
         mixed = {"value": v_mx, "speedup_vs_fp64": v_mx / value, "iterations": mx["iters"],
                  "macro_stress_rel_err_step1": err, "stress_field_rel_err_step1": ferr,
+                 "stress_field_rel_err_step1_median": float(np.median(dv)),
+                 "stress_field_rel_err_step1_p9999": float(np.quantile(dv, 0.9999)),
+                 "stress_field_voxels_above_1e-3": int((dv > 1e-3).sum()),
                  "ms_per_step": 1e3 * mx["wall"] / args.steps}
         log(f"[bench] mixed: {mx['iters']} iterations in {mx['wall']:.3f} s, <sigma> err {err:.2e}, "
             f"per-voxel stress err {ferr:.2e}")
