@@ -110,6 +110,18 @@ def run_form(V, R, mat, edot, args, precision, steps, warmup, profile, field=Fal
     return out
 
 
+def field_stats(a, b):
+    """Per voxel, the largest stress-component difference relative to the
+    max magnitude of b: maximum, median, 99.99th percentile and the number of
+    voxels above 1e-3 (at n=20 a few voxels of the FP64 form move by ~1e-3
+    under a 1e-12 relative change of the strain rate)."""
+    dv = np.abs(a - b).max(axis=1) / np.abs(b).max()
+    return {"stress_field_rel_err_step1": float(dv.max()),
+            "stress_field_rel_err_step1_median": float(np.median(dv)),
+            "stress_field_rel_err_step1_p9999": float(np.quantile(dv, 0.9999)),
+            "stress_field_voxels_above_1e-3": int((dv > 1e-3).sum())}
+
+
 def cpu_baseline(V, args, edot):
     """The oracle's fixed-point loop on a bounded sample of the workload."""
     import ctypes as C
@@ -242,6 +254,9 @@ def main():
     ap.add_argument("--dt", type=float, default=0.005)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--sensitivity", type=float, default=0.0,
+                    help="also run the FP64 form with the strain rate moved by this relative amount and report "
+                         "its per-voxel stress change beside the mixed form's error")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=96)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -326,21 +341,26 @@ def main():
         mx = run_form(V, R, mat, edot, args, V.MIXED, args.steps, args.warmup, profile=False, field=True)
         v_mx = mx["vox_iters"] / mx["wall"]
         err = float(np.linalg.norm(mx["avgs"][0] - f64["avgs"][0]) / np.linalg.norm(f64["avgs"][0]))
-        # per-voxel: each voxel's largest component difference relative to the
-        # field's max magnitude; its maximum, median, 99.99th percentile and
-        # the count above 1e-3 (at n=20 a few voxels of the FP64 form itself
-        # move by ~1e-3 under a 1e-12 relative change of the strain rate)
-        dv = np.abs(mx["sigma1"] - f64["sigma1"]).max(axis=1) / np.abs(f64["sigma1"]).max()
-        del mx["sigma1"], f64["sigma1"]
-        ferr = float(dv.max())
+        st = field_stats(mx["sigma1"], f64["sigma1"])
+        del mx["sigma1"]
         mixed = {"value": v_mx, "speedup_vs_fp64": v_mx / value, "iterations": mx["iters"],
-                 "macro_stress_rel_err_step1": err, "stress_field_rel_err_step1": ferr,
-                 "stress_field_rel_err_step1_median": float(np.median(dv)),
-                 "stress_field_rel_err_step1_p9999": float(np.quantile(dv, 0.9999)),
-                 "stress_field_voxels_above_1e-3": int((dv > 1e-3).sum()),
-                 "ms_per_step": 1e3 * mx["wall"] / args.steps}
+                 "macro_stress_rel_err_step1": err, **st, "ms_per_step": 1e3 * mx["wall"] / args.steps}
         log(f"[bench] mixed: {mx['iters']} iterations in {mx['wall']:.3f} s, <sigma> err {err:.2e}, "
-            f"per-voxel stress err {ferr:.2e}")
+            f"per-voxel stress err {st['stress_field_rel_err_step1']:.2e}")
+        if args.sensitivity > 0:
+            # the FP64 form against itself with the strain rate moved by a
+            # relative `sensitivity` (a seeded unit direction): the scale of
+            # per-voxel change the algorithm itself shows for such a change
+            u = np.random.default_rng(args.seed + 7).standard_normal(5)
+            ed2 = edot + args.sensitivity * u / np.linalg.norm(u)
+            log(f"[bench] FP64 form with the strain rate moved by {args.sensitivity:g}")
+            f2 = run_form(V, R, mat, ed2, args, V.FP64, 1, 0, profile=False, field=True)
+            mixed["fp64_sensitivity"] = {
+                "strain_rate_rel_change": args.sensitivity,
+                "macro_stress_rel_change_step1": float(np.linalg.norm(f2["avgs"][0] - f64["avgs"][0]) /
+                                                       np.linalg.norm(f64["avgs"][0])),
+                **field_stats(f2["sigma1"], f64["sigma1"])}
+        del f64["sigma1"]
 
     batched = None
     if not args.no_batched:
