@@ -487,6 +487,73 @@ __global__ void __launch_bounds__(256) k_y256(typename C2<T>::t* __restrict__ s,
   }
 }
 
+// y pass of length 512, in place: 512 = 16 x 32, with the 32-point stage as
+// 16 x 2.  Block = 16 consecutive kz columns of one (component, x) plane, 32
+// threads per column (kz fastest: 256-byte runs in FP64).
+//  1. thread t holds x[t + 32 j] (j < 16); a 16-point DFT over j and the
+//     twiddle W512^(t k1) give Y_t[k1];
+//  2. through LDS, thread u = 2 k1 + t0 gathers Y_{2 i + t0}[k1] (i < 16) and
+//     takes a 16-point DFT over i: A_t0[m];
+//  3. X[k1 + 16 m] = A_0[m] + W32^m A_1[m], X[k1 + 16 (m + 16)] = A_0[m] -
+//     W32^m A_1[m], the partner thread (u ^ 1) one shuffle away: thread u
+//     ends with X[k1 + 16 (m + 16 t0)] for m < 16.
+template <typename T>
+__global__ void __launch_bounds__(512) k_y512(typename C2<T>::t* __restrict__ s, FftGeom g,
+                                              const typename C2<T>::t* __restrict__ tw, int sign,
+                                              const int* __restrict__ done) {
+  using C = typename C2<T>::t;
+  if (done && *done) return;
+  __shared__ T buf[16 * 32 * 16];  // [k1][t][p], one part (real or imaginary) at a time
+  const int runs = (g.nzh + 15) >> 4;
+  const int run = (int)(blockIdx.x % runs);
+  const long plane = blockIdx.x / runs;  // c * nx + x
+  const int p = threadIdx.x & 15, t = threadIdx.x >> 4;
+  const int kz = run * 16 + p;
+  const bool active = kz < g.nzh;
+  C* base = s + plane * (long)g.ny * g.nzp + kz;
+  C v[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) v[j] = active ? base[(long)(t + 32 * j) * g.nzp] : cmake<T>(0, 0);
+  dft16<T>(v, sign);
+#pragma unroll
+  for (int k = 1; k < 16; k++) {  // W512^(t k1)
+    C w = tw[t * k];
+    if (sign > 0) w.y = -w.y;
+    v[k] = cmul(v[k], w);
+  }
+  // stage 2 gather: thread u = t holds k1 = u >> 1, t0 = u & 1
+  const int k1 = t >> 1, t0 = t & 1;
+  T re[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) buf[(k * 32 + t) * 16 + p] = v[k].x;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 16; i++) re[i] = buf[(k1 * 32 + 2 * i + t0) * 16 + p];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) buf[(k * 32 + t) * 16 + p] = v[k].y;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 16; i++) v[i] = cmake<T>(re[i], buf[(k1 * 32 + 2 * i + t0) * 16 + p]);
+  dft16<T>(v, sign);  // A_t0[m]
+  // stage 3: the partner (t ^ 1) is 16 lanes away in the same wave
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    C o;
+    o.x = __shfl_xor(v[m].x, 16);
+    o.y = __shfl_xor(v[m].y, 16);
+    C w = tw[16 * m];  // W32^m = W512^(16 m)
+    if (sign > 0) w.y = -w.y;
+    // t0 = 0 holds A_0 and receives A_1; t0 = 1 the reverse
+    const C a0 = t0 == 0 ? v[m] : o, a1 = cmul(t0 == 0 ? o : v[m], w);
+    v[m] = t0 == 0 ? cadd(a0, a1) : csub(a0, a1);
+  }
+  if (active) {
+#pragma unroll
+    for (int m = 0; m < 16; m++) base[(long)(k1 + 16 * (m + 16 * t0)) * g.nzp] = v[m];
+  }
+}
+
 // z passes of length 256 in the same 16 x 16 register form.  A pencil is
 // one (row, pair) of real components packed as f[2 pair] + i f[2 pair + 1]
 // (pair 2 holds component 4 alone); 16 pencils per block, thread t of a
@@ -1110,6 +1177,11 @@ static hipError_t launch_y_t(void* s, const FftGeom& g, const AxisPlan& ap, cons
   if (g.ny == 256 && Bq == 0) {  // the register 16 x 16 form
     const unsigned grid = (unsigned)(5L * g.nx * ((g.nzh + 15) / 16));
     hipLaunchKernelGGL((k_y256<T>), dim3(grid), dim3(256), 0, st, (C*)s, g, (const C*)tw, sign, done);
+    return hipGetLastError();
+  }
+  if (g.ny == 512 && Bq == 0) {  // the register 16 x 16 x 2 form
+    const unsigned grid = (unsigned)(5L * g.nx * ((g.nzh + 15) / 16));
+    hipLaunchKernelGGL((k_y512<T>), dim3(grid), dim3(512), 0, st, (C*)s, g, (const C*)tw, sign, done);
     return hipGetLastError();
   }
   const PassShape ps = pass_shape(sizeof(T) == 8 ? 0 : 1, kPassY, g, Bq);
