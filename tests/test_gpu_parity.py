@@ -107,7 +107,8 @@ def test_taylor_nr_batch_matches_oracle(gpu, vpfft, oracle, nexp):
 
 
 @pytest.mark.parametrize("dims", [(16, 16, 16), (8, 12, 10), (12, 10, 15), (24, 18, 20), (32, 16, 28), (4, 256, 10),
-                                  (6, 256, 256), (256, 4, 6), (256, 8, 36), (4, 512, 10), (3, 512, 40)])
+                                  (6, 256, 256), (256, 4, 6), (256, 8, 36), (4, 512, 10), (3, 512, 40),
+                                  (4, 6, 512)])
 def test_strain_rate_evolution_matches_oracle(gpu, vpfft, oracle, dims):
     """FFT + Green + inverse FFT (R2C/C2R with the special-plane operator
     G_eff) against the reference's C2C formulation."""
@@ -124,7 +125,7 @@ def test_strain_rate_evolution_matches_oracle(gpu, vpfft, oracle, dims):
     assert rel_err(d_gpu, d_ref) < 1e-12, rel_err(d_gpu, d_ref)
 
 
-@pytest.mark.parametrize("dims", [(16, 16, 16), (256, 4, 6), (256, 8, 36), (6, 256, 256), (4, 512, 10)])
+@pytest.mark.parametrize("dims", [(16, 16, 16), (256, 4, 6), (256, 8, 36), (6, 256, 256), (4, 512, 10), (4, 6, 512)])
 def test_strain_rate_evolution_mixed_precision(gpu, vpfft, oracle, dims):
     """The FP32 transforms of the mixed form (256-point register passes and
     the generic ones, ragged kz runs included) against the FP64 oracle."""
@@ -193,7 +194,7 @@ def test_strain_step_64cubed_512_grains_n20(gpu, vpfft, oracle):
     _check_step(avg, info, ref, f)
 
 
-@pytest.mark.parametrize("dims", [(256, 256, 4), (4, 8, 256), (4, 512, 8)])
+@pytest.mark.parametrize("dims", [(256, 256, 4), (4, 8, 256), (4, 512, 8), (4, 4, 512)])
 def test_strain_step_with_register_ffts(gpu, vpfft, oracle, dims):
     """Whole steps on grids with 256- and 512-point axes, so the register-form
     passes (x with the Green operator, y, and the z real/half-spectrum pair)
