@@ -116,3 +116,15 @@ def test_material_struct_roundtrip(vpfft):
     assert list(c.rate_sensitivity) == [10] * 12
     assert list(c.gamma_dot_base) == [15.5] * 12
     assert c.theta0[3] == 430.0 and c.hardening[143] == 1.0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus N > 1 exits with a message before any
+    GPU work."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in (r.stdout + r.stderr)
