@@ -277,7 +277,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one GPU: the whole voxel grid lives in one MI355X's HBM and there is no "
-              "multi-GPU decomposition in this build (--gpus must be 1)", file=sys.stderr)
+              "multi-GPU decomposition in this library (--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe_mib > 0:
