@@ -1,13 +1,17 @@
 """Benchmark: voxel-iterations/s of one VPFFT strain step on one MI355X.
 
-Workload (the headline configuration): 256^3 periodic-Voronoi polycrystal,
-4096 grains with Haar-random orientations, FCC, n=20, tau0=1, gdot0=1, a seeded
-random symmetric traceless strain rate of Frobenius norm 1, tolerance 1e-6,
-at most 100 fixed-point iterations, dt=0.005; consecutive strain steps (the
-reference's RunVPFFT loop).  value = N * iterations / wall time over the timed
-steps (plain FP64 form).  The mixed form (FP32 FFT + Green, FP64 Newton) runs
-the same steps from the same initial state and reports its speedup and its
-macroscopic-stress error against the FP64 form.
+Workload (the headline configuration): one strain step of a 256^3
+periodic-Voronoi polycrystal, 4096 grains with Haar-random orientations, FCC,
+n=20, tau0=1, gdot0=1, a seeded random symmetric traceless strain rate of
+Frobenius norm 1, tolerance 1e-6, at most 100 fixed-point iterations,
+dt=0.005.  Every timed step is that same step: the state it starts from is
+saved on the device once and restored (outside the timed region) before each
+repetition, so the whole step -- prologue, fixed-point loop, epilogue --
+runs every time on inputs resident in HBM.  value = N * iterations / wall
+time summed over the timed steps (plain FP64 form).  The mixed form (FP32 FFT
++ Green, FP64 Newton) runs the same step and reports its speedup and its
+macroscopic-stress error against the FP64 form.  --consecutive runs the
+reference's RunVPFFT loop instead (each step from the previous one's state).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] ...
 """
@@ -68,42 +72,48 @@ def measured_traffic(args, kernel):
 
 
 def run_form(V, R, mat, edot, args, precision, steps, warmup, profile, field=False):
-    """Warm up, then time `steps` strain steps from the initial state.  The
-    timed steps replay the fixed-point loop as captured HIP graphs.  With
-    profile=True the same steps run once more from the same initial state with
-    every kernel launched from the host between HIP events on the grid's
-    stream, which gives the per-kernel durations."""
+    """Warm up, then time `steps` strain steps.  The timed steps replay the
+    fixed-point loop as captured HIP graphs.  Unless --consecutive, every step
+    starts from the same saved initial state (restored on the device between
+    steps, untimed).  With profile=True, min(steps, --profile-steps) steps run
+    once more with every kernel launched from the host between HIP events on
+    the grid's stream, which gives the per-kernel durations."""
     n = R.shape[0]
     g = V.MaterialGrid(args.size, args.size, args.size, mat, precision=precision)
+    g.SetVoxelLength(1.0, 1.0, 1.0)
     g.SendDataAndInitialize(R)
-    for _ in range(warmup):
-        g.RunSingleStrainStep(edot, args.max_iter, args.dt, args.tol)
+    g.SaveState()
 
-    def timed(prof):
-        # restart from the initial state so every pass times the same steps
-        g.SendDataAndInitialize(R)
-        g.SetVoxelLength(1.0, 1.0, 1.0)
-        g.SetProfiling(prof)
-        iters, avgs, infos = 0, [], []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            avgs.append(g.RunSingleStrainStep(edot, args.max_iter, args.dt, args.tol))
+    def one(prof_steps=None, timed=True):
+        k = steps if prof_steps is None else prof_steps
+        iters, avgs, infos, wall = 0, [], [], 0.0
+        g.RestoreState()
+        for _ in range(k):
+            if not args.consecutive:
+                g.RestoreState()
+            t0 = time.perf_counter()
+            a = g.RunSingleStrainStep(edot, args.max_iter, args.dt, args.tol)
+            wall += time.perf_counter() - t0   # RunSingleStrainStep returns after the device is done
+            avgs.append(a)
             infos.append(g.last_info.as_dict())
             iters += g.last_info.iterations
-        wall = time.perf_counter() - t0
         return dict(wall=wall, iters=iters, vox_iters=n * iters, avgs=avgs, infos=infos)
 
-    out = timed(False)
+    g.SetProfiling(False)
+    for _ in range(warmup):
+        g.RestoreState()
+        g.RunSingleStrainStep(edot, args.max_iter, args.dt, args.tol)
+    out = one()
     if profile:
-        p = timed(True)
+        g.SetProfiling(True)
+        p = one(prof_steps=min(steps, args.profile_steps))
         out["ktimes"] = g.KernelTimes()
-        out["profiled_wall"] = p["wall"]
+        out["profiled_wall"] = p["wall"] / len(p["avgs"])
         out["profiled_same_bits"] = all(np.array_equal(a, b) for a, b in zip(out["avgs"], p["avgs"]))
+        g.SetProfiling(False)
     if field:
         # the stress field after the first step (untimed), for the per-voxel error
-        g.SendDataAndInitialize(R)
-        g.SetVoxelLength(1.0, 1.0, 1.0)
-        g.SetProfiling(False)
+        g.RestoreState()
         g.RunSingleStrainStep(edot, args.max_iter, args.dt, args.tol)
         out["sigma1"] = g.Stress()
     g.close()
@@ -150,8 +160,17 @@ def cpu_baseline(V, args, edot):
         lib.vo_voxel_update(C.byref(m), p(mall), p(crss), p(edot), p(L0), p(dtl), p(sig), p(lam), p(phi), p(S), p(mx))
         its += 1
     wall = time.perf_counter() - t0
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {
         "value": n * its / wall, "unit": "voxel-iters/s", "cores": O.num_threads(), "kind": "port",
+        "cpu_model": model, "nproc": os.cpu_count(),
         "sample": f"oracle fixed-point loop (FFT + literal Green + per-voxel NR) on a {s}^3 grid, "
                   f"{grains} grains, n={args.nexp}, same strain rate; {its} iterations in {wall:.1f} s; "
                   "prologue excluded",
@@ -254,6 +273,10 @@ def main():
     ap.add_argument("--dt", type=float, default=0.005)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--consecutive", action="store_true",
+                    help="time consecutive strain steps (RunVPFFT) instead of repetitions of the first step")
+    ap.add_argument("--profile-steps", type=int, default=3,
+                    help="steps of the host-launched pass that gives the per-kernel HIP-event times")
     ap.add_argument("--sensitivity", type=float, default=0.0,
                     help="also run the FP64 form with the strain rate moved by this relative amount and report "
                          "its per-voxel stress change beside the mixed form's error")
@@ -321,15 +344,19 @@ def main():
             b = fft_bytes(k, n, nc, 8)
             entry["GBps"] = b / (ms / cnt) / 1e6
         if k == "nr_update":
-            flops = n * f64["iters"] * NR_FLOPS_FIXED + nr_steps_total * NR_FLOPS_PER_STEP
-            entry["TFLOPs"] = flops / (ms / 1e3) / 1e12
+            # per launch (one fixed-point iteration over the grid): the fixed
+            # work of every voxel plus the Newton steps of an average iteration
+            flops = n * NR_FLOPS_FIXED + nr_steps_total / f64["iters"] * NR_FLOPS_PER_STEP
+            entry["flops_per_launch"] = flops
+            entry["TFLOPs"] = flops / (ms / cnt / 1e3) / 1e12
             entry["GBps"] = NR_BYTES * n / (ms / cnt) / 1e6
         per_kernel[k] = entry
     if dom == "nr_update":
         ach = per_kernel[dom]["TFLOPs"]
-        roof = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        roof = {"bound": "fp64_valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach / FP64_PEAK_TFLOPS, "traffic": measured_traffic(args, dom), "kernel": "nr_update",
-                "note": "FP64 vector ALU (no MFMA): MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s"}
+                "note": "FP64 vector ALU work (no MFMA): MI355X's FP64 vector and matrix peaks are both "
+                        "78.6 TFLOP/s"}
     else:
         ach = per_kernel[dom]["GBps"]
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -396,7 +423,10 @@ def main():
         "config": {
             "workload": f"one VPFFT strain step per step: {args.size}^3 periodic Voronoi, {args.grains} grains, "
                         f"FCC n={args.nexp}, random unit traceless strain rate, tol {args.tol}, "
-                        f"max {args.max_iter} iterations, dt {args.dt}",
+                        f"max {args.max_iter} iterations, dt {args.dt}; "
+                        + ("consecutive steps (RunVPFFT)" if args.consecutive else
+                           "every step from the same initial state (restored on the device, untimed)"),
+            "consecutive": bool(args.consecutive),
             "grid": args.size, "grains": args.grains, "rate_sensitivity": args.nexp, "tol": args.tol,
             "max_iter": args.max_iter, "iterations": f64["iters"], "precision": "fp64",
         },
@@ -406,7 +436,7 @@ def main():
         "nr_lane_efficiency": nr_steps_total / sum(i["nr_wave_steps_total"] for i in f64["infos"]),
         "taylor_steps_per_voxel": sum(i["taylor_steps_total"] for i in f64["infos"]) / (n * args.steps),
         "ms_prologue": sum(i["ms_prologue"] for i in f64["infos"]) / args.steps,
-        "ms_per_step_host_launched": 1e3 * f64["profiled_wall"] / args.steps,
+        "ms_per_step_host_launched": 1e3 * f64["profiled_wall"],
         "kernel_pass_same_bits": f64["profiled_same_bits"],
         "roofline": roof,
         "kernels": per_kernel,

@@ -69,8 +69,13 @@ typedef struct {
 
 /* Run the requests (any order; served by arrival time, in real time) to
  * completion.  results: count entries; stats may be NULL.  Returns VPFFT_OK
- * unless the loop itself failed (a request too large for the budget is
- * marked VPFFT_E_NOMEM in its result and skipped). */
+ * unless the loop itself failed.  A request that cannot be served is marked
+ * in its result and skipped, alone: VPFFT_E_NOMEM when it can never fit the
+ * budget, VPFFT_E_INVALID for bad fields (edge, grains, increments, arrival
+ * time, time step), VPFFT_E_NONFINITE / VPFFT_E_DEGENERATE for a non-finite
+ * or zero strain rate in any increment; a request whose step fails
+ * (degenerate voxels, non-finite values that an FP64 rerun does not cure)
+ * ends with that step's status. */
 int vpfft_serve(const vpfft_serve_config* cfg, const vpfft_request* requests, int count,
                 vpfft_request_result* results, vpfft_serve_stats* stats);
 

@@ -80,7 +80,7 @@ def load():
         f.argtypes = args
     lib.vo_get_max_rss.restype = C.c_double
     for name in ("vo_lu_solve_fullpiv", "vo_inverse", "vo_initialize", "vo_voxel_update", "vo_strain_step",
-                 "vo_num_threads"):
+                 "vo_num_threads", "vo_build_L0"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -252,6 +252,17 @@ def strain_rate_evolution(dims, material, L0, phi, lengths=(1.0, 1.0, 1.0)):
     out = np.empty_like(phi)
     load().vo_strain_rate_evolution(C.byref(mat), _p(_f(L0, (25,))), _p(phi), _p(out))
     return out
+
+
+def build_L0(dims, material, R, crss, sigma):
+    """MaterialGrid::BuildLinearRefMediumStiffness at the given stress field
+    (N x 5): (L0 column-major 25, rc; rc -1 = a singular secant modulus)."""
+    nx, ny, nz = dims
+    mat = make_material(nx, ny, nz, material)
+    m_all = np.ascontiguousarray(schmid_all(R).reshape(-1))
+    L0 = np.zeros(25)
+    rc = load().vo_build_L0(C.byref(mat), _p(m_all), _p(_f(crss).reshape(-1)), _p(_f(sigma).reshape(-1)), _p(L0))
+    return L0, rc
 
 
 def schmid_all(R):

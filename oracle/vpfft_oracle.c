@@ -691,12 +691,55 @@ void vo_fft3d(double *data, int nx, int ny, int nz, int sign) {
   fft_axis(data, nx, ny, nz, 0, sign);
 }
 
-/* MaterialGrid::StrainRateEvolutionWithFFT, MaterialGrid.cpp:624-761.
- * phi: 5 real components, each N long (component-major).  dtilde receives
- * Re(IFFT(-S5 * FFT(phi))) / N, the only part the reference reads
- * (MaterialGrid.h:354-363). */
-void vo_strain_rate_evolution(const vo_material *mat, const double L0[25],
-                              const double *phi, double *dtilde) {
+/* The symmetrised 5x5 Green operator at frequency index v of the C2C grid
+ * (MaterialGrid.cpp:682-731): 0 for the DC term (:733-740), the special-plane
+ * operator Sspec for an index N/2 or N/2+1 along an even axis (:710-713),
+ * else vo_green_sym5 at the normalised wave vector.  Returns 0 (DC) or 1. */
+static int green_at(const vo_material *mat, const double L0[25], const double Sspec[25], long v, double S5[25]) {
+  const int nx = mat->nx, ny = mat->ny, nz = mat->nz;
+  const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
+  const int ex = nx % 2 == 0, ey = ny % 2 == 0, ez = nz % 2 == 0;
+  int ix = (int)(v / ((long)ny * nz)), iy = (int)((v / nz) % ny), iz = (int)(v % nz);
+  if (ix == 0 && iy == 0 && iz == 0) return 0;
+  if ((ex && (ix == cx || ix == cx + 1)) || (ey && (iy == cy || iy == cy + 1)) ||
+      (ez && (iz == cz || iz == cz + 1))) {
+    memcpy(S5, Sspec, sizeof(double) * 25);
+  } else {
+    double k[3] = {(double)ix, (double)iy, (double)iz};
+    if (ix > cx) k[0] -= nx;
+    if (iy > cy) k[1] -= ny;
+    if (iz > cz) k[2] -= nz;
+    k[0] /= (nx * mat->lx);
+    k[1] /= (ny * mat->ly);
+    k[2] /= (nz * mat->lz);
+    /* SVector3::Normalize with its float sqrtf, LinearAlgebra.cpp:545-559 */
+    double len = (double)sqrtf((float)(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]));
+    k[0] /= len; k[1] /= len; k[2] /= len;
+    vo_green_sym5(L0, k, S5);
+  }
+  return 1;
+}
+
+/* The operator of every frequency for one L0 and one set of voxel lengths
+ * (N x 25, DC rows zero), or NULL when it does not fit in memory.  Within a
+ * strain step the reference rebuilds exactly these values every iteration. */
+static double *green_table(const vo_material *mat, const double L0[25]) {
+  const long N = (long)mat->nx * mat->ny * mat->nz;
+  double *T = (double *)malloc(sizeof(double) * 25 * N);
+  if (!T) return NULL;
+  double Sspec[25];
+  vo_special_sym5(L0, Sspec);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+  for (long v = 0; v < N; v++)
+    if (!green_at(mat, L0, Sspec, v, T + 25 * v)) memset(T + 25 * v, 0, sizeof(double) * 25);
+  return T;
+}
+
+/* StrainRateEvolutionWithFFT with the operator from table (NULL: computed). */
+static void srev(const vo_material *mat, const double L0[25], const double *table, const double *phi,
+                 double *dtilde) {
   const int nx = mat->nx, ny = mat->ny, nz = mat->nz;
   const long N = (long)nx * ny * nz;
   double *P[5], *D[5];
@@ -707,34 +750,19 @@ void vo_strain_rate_evolution(const vo_material *mat, const double L0[25],
     vo_fft3d(P[c], nx, ny, nz, -1);
   }
   double Sspec[25];
-  vo_special_sym5(L0, Sspec);
-  const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
-  const int ex = nx % 2 == 0, ey = ny % 2 == 0, ez = nz % 2 == 0;
+  if (!table) vo_special_sym5(L0, Sspec);
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 64)
 #endif
   for (long v = 0; v < N; v++) {
-    int ix = (int)(v / ((long)ny * nz)), iy = (int)((v / nz) % ny), iz = (int)(v % nz);
-    if (ix == 0 && iy == 0 && iz == 0) {
+    double S5l[25];
+    const double *S5 = S5l;
+    if (table) {
+      S5 = table + 25 * v;
+      if (v == 0) { for (int c = 0; c < 5; c++) { D[c][0] = 0; D[c][1] = 0; } continue; }
+    } else if (!green_at(mat, L0, Sspec, v, S5l)) {
       for (int c = 0; c < 5; c++) { D[c][0] = 0; D[c][1] = 0; }
       continue;
-    }
-    double S5[25];
-    if ((ex && (ix == cx || ix == cx + 1)) || (ey && (iy == cy || iy == cy + 1)) ||
-        (ez && (iz == cz || iz == cz + 1))) {
-      memcpy(S5, Sspec, sizeof S5);
-    } else {
-      double k[3] = {(double)ix, (double)iy, (double)iz};
-      if (ix > cx) k[0] -= nx;
-      if (iy > cy) k[1] -= ny;
-      if (iz > cz) k[2] -= nz;
-      k[0] /= (nx * mat->lx);
-      k[1] /= (ny * mat->ly);
-      k[2] /= (nz * mat->lz);
-      /* SVector3::Normalize with its float sqrtf, LinearAlgebra.cpp:545-559 */
-      double len = (double)sqrtf((float)(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]));
-      k[0] /= len; k[1] /= len; k[2] /= len;
-      vo_green_sym5(L0, k, S5);
     }
     for (int i = 0; i < 5; i++) {
       double re = 0, im = 0;
@@ -753,6 +781,15 @@ void vo_strain_rate_evolution(const vo_material *mat, const double L0[25],
     free(P[c]);
     free(D[c]);
   }
+}
+
+/* MaterialGrid::StrainRateEvolutionWithFFT, MaterialGrid.cpp:624-761.
+ * phi: 5 real components, each N long (component-major).  dtilde receives
+ * Re(IFFT(-S5 * FFT(phi))) / N, the only part the reference reads
+ * (MaterialGrid.h:354-363). */
+void vo_strain_rate_evolution(const vo_material *mat, const double L0[25],
+                              const double *phi, double *dtilde) {
+  srev(mat, L0, NULL, phi, dtilde);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -796,13 +833,17 @@ int vo_initialize(const vo_material *mat, const double *m_all, const double *crs
 }
 
 /* MaterialGrid::BuildLinearRefMediumStiffness, MaterialGrid.cpp:219-282
- * (single rank: the slab weighting multiplies and divides by Nx). */
-void vo_build_L0(const vo_material *mat, const double *m_all, const double *crss,
-                 const double *sigma, double L0[25]) {
+ * (single rank: the slab weighting multiplies and divides by Nx).
+ * Returns 0, or -1 when a voxel's secant modulus is exactly singular: the
+ * reference's Eigen inverse (:248) has no answer there, and averaging the
+ * un-inverted modulus would hide it.  L0 is then left untouched. */
+int vo_build_L0(const vo_material *mat, const double *m_all, const double *crss,
+                const double *sigma, double L0[25]) {
   const long N = nvox(mat);
   double *inv = (double *)malloc(sizeof(double) * 25 * N);
+  int bad = 0;
 #ifdef _OPENMP
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) reduction(| : bad)
 #endif
   for (long v = 0; v < N; v++) {
     const double *m = m_all + 60 * v;
@@ -817,8 +858,12 @@ void vo_build_L0(const vo_material *mat, const double *m_all, const double *crss
       for (int j = 0; j < 5; j++)
         for (int i = 0; i < 5; i++) M[j * 5 + i] += m[5 * k + i] * m[5 * k + j] * g / tau[k];
     }
-    vo_inverse(5, M);
+    if (vo_inverse(5, M) != 0) bad = 1;
     memcpy(inv + 25 * v, M, sizeof M);
+  }
+  if (bad) {
+    free(inv);
+    return -1;
   }
   double A[25];
   memset(A, 0, sizeof A);
@@ -831,6 +876,7 @@ void vo_build_L0(const vo_material *mat, const double *m_all, const double *crss
     if (fabs(A[e]) < 1e-10) A[e] = 0;     /* NormalizeValues, LinearAlgebra.h:605-613 */
     L0[e] = A[e];
   }
+  return 0;
 }
 
 /* Per-voxel body of the fixed-point loop, MaterialGrid.cpp:946-1003.
@@ -943,17 +989,23 @@ int vo_strain_step(vo_material *mat, double *R, double *crss, double *acc_shear,
     return -1;
   }
   double L0[25];
-  vo_build_L0(mat, m_all, crss, sigma, L0);
+  if (vo_build_L0(mat, m_all, crss, sigma, L0) != 0) {
+    info->status = -2;
+    free(m_all);
+    return -2;
+  }
   memcpy(info->L0, L0, sizeof L0);
   int steps = 0;
   double es = 0, ee = 0, avg[5] = {0, 0, 0, 0, 0}, avge[5] = {0, 0, 0, 0, 0};
   const double nd = (double)(mat->nx * mat->ny * mat->nz);
+  double *table = green_table(mat, L0);   /* same values every iteration; NULL: recomputed */
   do {
-    vo_strain_rate_evolution(mat, L0, phi, dtilde);
+    srev(mat, L0, table, phi, dtilde);
     double S[12], mx = 0;
     if (vo_voxel_update(mat, m_all, crss, edot, L0, dtilde, sigma, lambda, phi, S, &mx) != 0) {
       info->status = -1;
       free(m_all);
+      free(table);
       return -1;
     }
     es = S[10] / nd;
@@ -970,6 +1022,7 @@ int vo_strain_step(vo_material *mat, double *R, double *crss, double *acc_shear,
   memcpy(info->avg_stress, avg, sizeof avg);
   memcpy(info->avg_strain_rate, avge, sizeof avge);
   free(m_all);
+  free(table);
   if (do_epilogue) vo_epilogue(mat, R, crss, acc_shear, sigma, edot, dt);
   return 0;
 }

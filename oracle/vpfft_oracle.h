@@ -37,7 +37,7 @@ typedef struct {
 
 typedef struct {
   int iterations;
-  int status;                   /* 0 ok, -1 degenerate (max RSS < 1e-10) */
+  int status;                   /* 0 ok, -1 degenerate (max RSS < 1e-10), -2 singular secant modulus */
   double err_stress, err_strain_rate;
   double avg_stress[5], avg_strain_rate[5];
   double L0[25];
@@ -85,8 +85,9 @@ void vo_strain_rate_evolution(const vo_material *mat, const double L0[25],
 /* ---- grid-level phases ---- */
 int vo_initialize(const vo_material *mat, const double *m_all, const double *crss,
                   const double edot[5], double *sigma, double *lambda, double *phi);
-void vo_build_L0(const vo_material *mat, const double *m_all, const double *crss,
-                 const double *sigma, double L0[25]);
+/* 0, or -1 when a voxel's secant modulus is exactly singular (L0 untouched) */
+int vo_build_L0(const vo_material *mat, const double *m_all, const double *crss,
+                const double *sigma, double L0[25]);
 int vo_voxel_update(const vo_material *mat, const double *m_all, const double *crss,
                     const double edot[5], const double L0[25], const double *dtilde,
                     double *sigma, double *lambda, double *phi, double *sums12,

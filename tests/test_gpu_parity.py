@@ -344,3 +344,36 @@ def test_graph_loop_matches_host_launched_loop(gpu, vpfft, monkeypatch, graph_it
         assert s0[1] == s1[1]
         for x0, x1 in zip(s0[:1] + s0[2:], s1[:1] + s1[2:]):
             assert np.array_equal(x0, x1)
+
+
+def test_laminate_n1_fixed_point_closed_form(gpu, vpfft, oracle):
+    """The n = 1 two-phase laminate on a 9 x 3 x 3 grid (odd edges: the
+    generic radix-3 passes, no special planes): the GPU's converged stress
+    is the closed-form fixed point of the reference's loop (each phase
+    M_P s_P = E + G (s_P - <s>) / 2, see test_oracle_closed_forms), with L0
+    the volume average of the phases' inverse moduli -- checked without the
+    oracle's loop."""
+    from helpers import random_rotation
+    from test_oracle_closed_forms import _colmajor, _normalized, _secant, laminate_code
+    rng = np.random.default_rng(12)
+    RA, RB = random_rotation(rng), random_rotation(rng)
+    dims = (9, 3, 3)
+    N = 81
+    layerA = np.array([v // 9 < 4 for v in range(N)])
+    fA = 4.0 / 9.0
+    R = np.where(layerA[:, None], RA.reshape(1, 9), RB.reshape(1, 9))
+    edot = random_unit_edot(4)
+    with vpfft.MaterialGrid(*dims, vpfft.Material.uniform(1.0, 1)) as g:
+        g.SendDataAndInitialize(R)
+        avg = g.RunSingleStrainStep(edot, 2000, 0.0, 1e-13)
+        info = g.last_info
+        sigma = g.Stress()
+    assert info.converged == 1 and info.iterations < 2000
+    KA = np.linalg.inv(_secant(oracle, RA, np.zeros(5), np.ones(12), 1))
+    KB = np.linalg.inv(_secant(oracle, RB, np.zeros(5), np.ones(12), 1))
+    L0 = _colmajor(_normalized(fA * KA + (1 - fA) * KB))
+    assert rel_err(np.array(info.L0), L0) < 1e-12
+    sA, sB = laminate_code(oracle, RA, RB, fA, edot, L0)
+    want = np.where(layerA[:, None], sA, sB)
+    assert rel_err(sigma, want) < 1e-10, rel_err(sigma, want)
+    assert rel_err(avg, fA * sA + (1 - fA) * sB) < 1e-10

@@ -388,6 +388,14 @@ class MaterialGrid:
     def SetPrecision(self, precision: int):
         _check(self._lib.vpfft_grid_set_precision(self._h, int(precision)))
 
+    def SaveState(self):
+        """Device copy of the state a strain step changes (vpfft_grid_save_state)."""
+        _check(self._lib.vpfft_grid_save_state(self._h))
+
+    def RestoreState(self):
+        """Back to the last SaveState (vpfft_grid_restore_state)."""
+        _check(self._lib.vpfft_grid_restore_state(self._h))
+
     # ---- data
     def SendDataAndInitialize(self, Orientations, Stress=None, CRSS=None):
         R = _f64(Orientations, (self.N, 9))

@@ -34,6 +34,8 @@ def build(verbose: bool = False) -> str:
     ]
     outdir = os.path.join(HERE, "_build")
     os.makedirs(outdir, exist_ok=True)
+    # the translation units are independent: compile them side by side
+    procs = []
     for src in SOURCES:
         obj = os.path.join(outdir, src + ".o")
         cmd = [hipcc()] + common + ["-c", os.path.join(CSRC, src), "-o", obj]
@@ -42,8 +44,11 @@ def build(verbose: bool = False) -> str:
                    os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
         objs.append(obj)
+    failed = [cmd for cmd, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"]
     if verbose:

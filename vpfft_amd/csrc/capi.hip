@@ -117,6 +117,8 @@ struct Grid {
   IterRecord* drec = nullptr;
   int rec_cap = 0;
   PrologueRecord* dprol = nullptr;
+  HealthRecord* dhealth = nullptr;
+  HealthRecord* hhealth = nullptr;
   StepConst* hC = nullptr;
   IterRecord* hrec = nullptr;
   PrologueRecord* hprol = nullptr;
@@ -397,8 +399,8 @@ static size_t grid_bytes(const vpfft_grid_desc* d) {
   const long nc = (long)d->nx * d->ny * padded_half(d->nz);
   size_t b = sizeof(double) * 42 * (size_t)n + sizeof(double2) * 5 * (size_t)nc;
   b += sizeof(StepConst) + sizeof(GreenConst<double>) + sizeof(GreenConst<float>) + sizeof(PrologueRecord) +
-       sizeof(LoopState) + sizeof(IterRecord) * 128;
-  b += sizeof(double) * (size_t)std::max<long>(4096L * 18, nr_partial_rows(n) * kPartialStride);
+       sizeof(HealthRecord) + sizeof(LoopState) + sizeof(IterRecord) * 128;
+  b += sizeof(double) * (size_t)std::max<long>(2L * 4096 * 18, nr_partial_rows(n) * kPartialStride);
   b += sizeof(double) * (size_t)std::min<long>(60L * n, 8L << 20);
   b += (sizeof(double2) + sizeof(float2)) * (size_t)(d->nx + d->ny + d->nz);
   return b;
@@ -494,8 +496,9 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
   ALLOC(g->dC, sizeof(StepConst));
   ALLOC(g->dG64, sizeof(GreenConst<double>));
   ALLOC(g->dG32, sizeof(GreenConst<float>));
-  ALLOC(g->partial, sizeof(double) * std::max<long>(4096L * 18, nr_partial_rows(n) * kPartialStride));
+  ALLOC(g->partial, sizeof(double) * std::max<long>(2L * 4096 * 18, nr_partial_rows(n) * kPartialStride));
   ALLOC(g->dprol, sizeof(PrologueRecord));
+  ALLOC(g->dhealth, sizeof(HealthRecord));
   ALLOC(g->dls, sizeof(LoopState));
   g->staging_elems = std::min<long>(60L * n, 8L << 20);
   ALLOC(g->staging, sizeof(double) * g->staging_elems);
@@ -506,6 +509,9 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
   g->hprol = (PrologueRecord*)p;
   if ((rc = halloc(g, &p, sizeof(LoopState)))) { grid_free(g); return rc; }
   g->hls = (LoopState*)p;
+  if ((rc = halloc(g, &p, sizeof(HealthRecord)))) { grid_free(g); return rc; }
+  g->hhealth = (HealthRecord*)p;
+  std::memset(g->hhealth, 0, sizeof(HealthRecord));
   if ((rc = ensure_records(g, 128))) { grid_free(g); return rc; }
   for (int a = 0; a < 3; a++)
     if ((rc = make_twiddles(g, a, dims[a]))) { grid_free(g); return rc; }
@@ -522,13 +528,48 @@ static int enqueue_iteration(Grid* g, int prec, double tol, int max_iter) {
   if ((rc = run_fft_green(g, prec, done))) return rc;
   kbegin(g, K_NR);
   HIP_TRY(launch_nr(prec, g->nexp_uniform, g->crss_uniform, g->n, g->dtil, g->phi, g->sigma, g->lambda, g->R, g->crss,
-                    g->dC, g->partial, done, g->st));
+                    g->dC, g->partial, g->dls, g->dhealth, g->st));
   kend(g, K_NR);
   g->phi_prec = prec;
   kbegin(g, K_ITER_FINALIZE);
   HIP_TRY(launch_iter_finalize(g->n, g->partial, g->dC, tol, max_iter, g->dls, g->drec, g->st));
   kend(g, K_ITER_FINALIZE);
   return VPFFT_OK;
+}
+
+// Cholesky factorisation of the symmetric part of a column-major 5x5:
+// true when every pivot is positive (the matrix is positive definite).
+static bool spd5(const double A[25]) {
+  double L[5][5] = {};
+  for (int j = 0; j < 5; j++) {
+    double d = A[j * 5 + j];
+    for (int k = 0; k < j; k++) d -= L[j][k] * L[j][k];
+    if (!(d > 0) || !std::isfinite(d)) return false;
+    L[j][j] = std::sqrt(d);
+    for (int i = j + 1; i < 5; i++) {
+      double t = 0.5 * (A[j * 5 + i] + A[i * 5 + j]);
+      for (int k = 0; k < j; k++) t -= L[i][k] * L[j][k];
+      L[i][j] = t / L[j][j];
+    }
+  }
+  return true;
+}
+
+// The status and message of a health record that has its flag set.
+static int health_error(const Grid* g, const HealthRecord& h, const char* where) {
+  const int code = (h.kernel == HK_PROLOGUE && h.field == HF_SECANT_INVERSE && std::isfinite(h.value) && h.value == 0.0)
+                       ? VPFFT_E_DEGENERATE
+                       : VPFFT_E_NONFINITE;
+  if (h.voxel >= 0) {
+    const long yz = (long)g->ny * g->nz;
+    const long x = h.voxel / yz, y = (h.voxel / g->nz) % g->ny, z = h.voxel % g->nz;
+    char it[32] = "";
+    if (h.iter > 0) std::snprintf(it, sizeof it, ", iteration %d", h.iter);
+    return set_error(code, "%s: %s %s at voxel %ld (x=%ld, y=%ld, z=%ld)%s (value %g)", where,
+                     health_kernel_name(h.kernel), health_field_name(h.field), h.voxel, x, y, z, it, h.value);
+  }
+  return set_error(code, "%s: %s %s (value %g)", where, health_kernel_name(h.kernel), health_field_name(h.field),
+                   h.value);
 }
 
 static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, double tol, double avg[5],
@@ -549,22 +590,29 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
   const int nb = point_blocks(n);
   fill_host_const(g, edot, dt);
   HIP_TRY(hipMemcpyAsync(g->dC, g->hC, sizeof(StepConst), hipMemcpyHostToDevice, g->st));
+  HIP_TRY(launch_health_reset(g->dhealth, g->st));
   HIP_TRY(hipEventRecord(g->ev[0], g->st));
   // prologue: MaterialGrid.cpp:903-913
   kbegin(g, K_PROLOGUE);
-  HIP_TRY(launch_prologue(prec, n, g->R, g->crss, g->dC, g->phi, g->sigma, g->lambda, g->partial, g->st));
+  HIP_TRY(launch_prologue(prec, n, g->R, g->crss, g->dC, g->phi, g->sigma, g->lambda, g->partial, g->dhealth, g->st));
   kend(g, K_PROLOGUE);
   g->phi_prec = prec;
   kbegin(g, K_L0_FINALIZE);
-  HIP_TRY(launch_L0_finalize(n, nb, g->partial, g->dC, g->dG64, g->dG32, g->dprol, g->st));
+  HIP_TRY(launch_L0_finalize(n, nb, g->partial, g->dC, g->dG64, g->dG32, g->dprol, g->dhealth, g->st));
   kend(g, K_L0_FINALIZE);
   HIP_TRY(hipMemcpyAsync(g->hprol, g->dprol, sizeof(PrologueRecord), hipMemcpyDeviceToHost, g->st));
+  HIP_TRY(hipMemcpyAsync(g->hhealth, g->dhealth, sizeof(HealthRecord), hipMemcpyDeviceToHost, g->st));
   HIP_TRY(hipEventRecord(g->ev[1], g->st));
   HIP_TRY(hipStreamSynchronize(g->st));
   kcollect(g);
   if (g->hprol->degenerate > 0)
     return set_error(VPFFT_E_DEGENERATE, "%.0f voxels with max resolved shear stress < 1e-10 (initialization)",
                      g->hprol->degenerate);
+  if (g->hhealth->flag) return health_error(g, *g->hhealth, "initialization");
+  // the reference medium is an average of positive definite inverses: a
+  // medium that is not positive definite means the prologue went wrong
+  if (!spd5(g->hprol->L0) || !spd5(g->hprol->L0inv))
+    return set_error(VPFFT_E_NONFINITE, "reference medium L0 (or its inverse) is not symmetric positive definite");
   // fixed-point loop: MaterialGrid.cpp:930-1068
   HIP_TRY(launch_loop_reset(g->dls, g->st));
   int it = 0;
@@ -623,11 +671,15 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
   }
   HIP_TRY(hipEventRecord(g->ev[2], g->st));
   const IterRecord& last = g->hrec[it - 1];
+  HIP_TRY(hipMemcpyAsync(g->hhealth, g->dhealth, sizeof(HealthRecord), hipMemcpyDeviceToHost, g->st));
+  HIP_TRY(hipStreamSynchronize(g->st));
+  const bool loop_clean = !g->hhealth->flag;
   // epilogue: MaterialGrid.cpp:1074-1079 (skipped for dt <= 0, where the
-  // reference's hardening divides 0 by 0)
-  if (dt > 0) {
+  // reference's hardening divides 0 by 0, and after a non-finite iteration:
+  // the kernel returns at entry then)
+  if (dt > 0 && loop_clean) {
     kbegin(g, K_EPILOGUE);
-    HIP_TRY(launch_epilogue(n, g->R, g->crss, g->acc, g->sigma, g->dC, g->st));
+    HIP_TRY(launch_epilogue(n, g->R, g->crss, g->acc, g->sigma, g->dC, g->dhealth, g->st));
     kend(g, K_EPILOGUE);
     g->crss_uniform = g->crss_uniform && g->hardening_uniform;
     double d[3];
@@ -637,9 +689,10 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
     g->lz *= (1.0 + d[2] * dt);
   }
   HIP_TRY(hipEventRecord(g->ev[3], g->st));
+  HIP_TRY(hipMemcpyAsync(g->hhealth, g->dhealth, sizeof(HealthRecord), hipMemcpyDeviceToHost, g->st));
   HIP_TRY(hipStreamSynchronize(g->st));
   kcollect(g);
-  bool finite = true;
+  bool finite = !g->hhealth->flag;
   for (int i = 0; i < 5; i++) finite = finite && std::isfinite(last.avg_stress[i]);
   finite = finite && std::isfinite(last.err_stress) && std::isfinite(last.err_strain_rate);
   if (avg)
@@ -672,7 +725,10 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
     info->ms_epilogue = ms;
     info->status = finite ? VPFFT_OK : VPFFT_E_NONFINITE;
   }
-  if (!finite) return set_error(VPFFT_E_NONFINITE, "non-finite average stress or error norm");
+  if (!finite) {
+    if (g->hhealth->flag) return health_error(g, *g->hhealth, loop_clean ? "epilogue" : "fixed-point loop");
+    return set_error(VPFFT_E_NONFINITE, "non-finite average stress or error norm");
+  }
   return VPFFT_OK;
 }
 
@@ -803,6 +859,7 @@ static void fill_vox_job(Grid* g, LoopState* ls, VoxJob* J) {
   J->ls = ls;
   J->rec = g->drec;
   J->prol = g->dprol;
+  J->health = g->dhealth;
 }
 
 static long pass_blocks(const Grid* g, int kind, int B) {
@@ -866,7 +923,7 @@ static int choose_B(Batch* b, Grid* g, FftPass pass, int job) {
 }
 
 // Group the grids of a step into launches and upload the tables.
-static int batch_plan(Batch* b, Grid** gs, int count, const double* dt) {
+static int batch_plan(Batch* b, Grid** gs, int count, const double* dt, const std::vector<char>& active) {
   b->launches.clear();
   int ids_at = 0, off_at = 0;
   auto add_group = [&](int kind, int prec, int B, int E, int nexp, int uniform, const std::vector<int>& members) {
@@ -890,6 +947,7 @@ static int batch_plan(Batch* b, Grid** gs, int count, const double* dt) {
   std::map<std::tuple<int, int, int>, std::vector<int>> nr;
   std::vector<int> epi;
   for (int j = 0; j < count; j++) {
+    if (!active[j]) continue;
     nr[std::make_tuple(gs[j]->prec, gs[j]->nexp_uniform, gs[j]->crss_uniform ? 1 : 0)].push_back(j);
     if (dt[j] > 0) epi.push_back(j);
   }
@@ -899,6 +957,7 @@ static int batch_plan(Batch* b, Grid** gs, int count, const double* dt) {
     const FftPass pass = (kind == L_ZF || kind == L_ZI) ? kPassZ : ((kind == L_YF || kind == L_YI) ? kPassY : kPassX);
     std::map<std::tuple<int, int, int>, std::vector<int>> fg;
     for (int j = 0; j < count; j++) {
+      if (!active[j]) continue;
       const int B = choose_B(b, gs[j], pass, j);
       if (B < 0) return set_error(VPFFT_E_DEVICE, "FFT block-shape tuning failed");
       const PassShape ps = pass_shape(gs[j]->prec, pass, gs[j]->geom, B);
@@ -911,10 +970,12 @@ static int batch_plan(Batch* b, Grid** gs, int count, const double* dt) {
     add_group(L_NR, std::get<0>(kv.first), 0, 0, std::get<1>(kv.first), std::get<2>(kv.first), kv.second);
   // epilogue for the grids that move in time (MaterialGrid.cpp:1074-1079)
   if (!epi.empty()) add_group(L_EPI, 0, 0, 0, 0, 0, epi);
-  // the one-block-per-grid kernels use the identity list
-  for (int j = 0; j < count; j++) b->hids[ids_at + j] = j;
-  b->launches.push_back(BLaunch{-1, 0, 0, 0, 0, 0, 0, 0, ids_at, off_at, count});
-  ids_at += count;
+  // the one-block-per-grid kernels (the convergence test) take the active grids
+  int nact = 0;
+  for (int j = 0; j < count; j++)
+    if (active[j]) b->hids[ids_at + nact++] = j;
+  b->launches.push_back(BLaunch{-1, 0, 0, 0, 0, 0, 0, 0, ids_at, off_at, nact});
+  ids_at += nact;
   if (ids_at > b->pool_cap - 8 || off_at > b->pool_cap - 8) return set_error(VPFFT_E_INVALID, "batch plan overflow");
   HIP_TRY(hipMemcpyAsync(b->dids, b->hids, sizeof(int) * ids_at, hipMemcpyHostToDevice, b->st));
   HIP_TRY(hipMemcpyAsync(b->doff, b->hoff, sizeof(long) * std::max(off_at, 1), hipMemcpyHostToDevice, b->st));
@@ -940,31 +1001,81 @@ static int batch_iteration(Batch* b, double tol, int max_iter) {
   return VPFFT_OK;
 }
 
+// Fill infos[j] for a grid that did not take part in the step.
+static void info_failed(vpfft_step_info* infos, double* avg, int j, int status, int prec) {
+  if (avg)
+    for (int i = 0; i < 5; i++) avg[5 * j + i] = std::nan("");
+  if (infos) {
+    std::memset(infos + j, 0, sizeof(vpfft_step_info));
+    infos[j].status = status;
+    infos[j].precision = prec;
+  }
+}
+
+// One strain step of count grids.  A grid whose input is bad (non-finite or
+// zero strain rate, non-finite time step, not uploaded) or whose prologue
+// fails (degenerate voxels, singular secant modulus, non-finite values) gets
+// its status in infos[j] (avg NaN) and sits out the rest of the step; the
+// others run as usual.  Returns the first non-zero per-grid status.
 static int batch_step(Batch* b, Grid** gs, int count, const double* edot, int max_iter, const double* dt,
                       double tol, double* avg, vpfft_step_info* infos) {
   if (count < 1 || !gs || !edot || !dt) return set_error(VPFFT_E_INVALID, "empty batch");
-  if (max_iter < 1) return set_error(VPFFT_E_INVALID, "max_iter must be >= 1");
-  if (!(tol >= 0)) return set_error(VPFFT_E_INVALID, "bad tolerance");
+  auto fail_all = [&](int code) {
+    for (int j = 0; j < count; j++) info_failed(infos, avg, j, code, gs[j] ? gs[j]->prec : 0);
+    return code;
+  };
+  if (max_iter < 1) return fail_all(set_error(VPFFT_E_INVALID, "max_iter must be >= 1"));
+  if (!(tol >= 0)) return fail_all(set_error(VPFFT_E_INVALID, "bad tolerance"));
   for (int j = 0; j < count; j++) {
-    if (!gs[j]) return set_error(VPFFT_E_INVALID, "null grid %d", j);
-    if (!std::isfinite(dt[j])) return set_error(VPFFT_E_INVALID, "grid %d: bad time step", j);
-    if (!gs[j]->uploaded) return set_error(VPFFT_E_STATE, "grid %d: run before upload", j);
+    if (!gs[j]) return fail_all(set_error(VPFFT_E_INVALID, "null grid %d", j));
     for (int k = 0; k < j; k++)
-      if (gs[k] == gs[j]) return set_error(VPFFT_E_INVALID, "grid %d appears twice", j);
+      if (gs[k] == gs[j]) return fail_all(set_error(VPFFT_E_INVALID, "grid %d appears twice", j));
+  }
+  std::vector<int> status(count, VPFFT_OK);
+  int first = VPFFT_OK;
+  auto grid_fail = [&](int j, int code) {
+    status[j] = code;
+    if (!first) first = code;
+  };
+  for (int j = 0; j < count; j++) {
+    if (!gs[j]->uploaded) {
+      grid_fail(j, set_error(VPFFT_E_STATE, "grid %d: run before upload", j));
+      continue;
+    }
+    if (!std::isfinite(dt[j])) {
+      grid_fail(j, set_error(VPFFT_E_INVALID, "grid %d: bad time step", j));
+      continue;
+    }
     double ee = 0;
+    bool fin = true;
     for (int i = 0; i < 5; i++) {
-      if (!std::isfinite(edot[5 * j + i])) return set_error(VPFFT_E_NONFINITE, "grid %d: non-finite strain rate", j);
+      fin = fin && std::isfinite(edot[5 * j + i]);
       ee += edot[5 * j + i] * edot[5 * j + i];
     }
-    if (!(ee > 0)) return set_error(VPFFT_E_DEGENERATE, "grid %d: zero strain rate", j);
+    if (!fin) grid_fail(j, set_error(VPFFT_E_NONFINITE, "grid %d: non-finite strain rate", j));
+    else if (!(ee > 0)) grid_fail(j, set_error(VPFFT_E_DEGENERATE, "grid %d: zero strain rate", j));
+  }
+  std::vector<char> active(count);
+  int nact = 0;
+  for (int j = 0; j < count; j++) nact += (active[j] = status[j] == VPFFT_OK);
+  auto finish_failed = [&]() {
+    for (int j = 0; j < count; j++)
+      if (status[j]) info_failed(infos, avg, j, status[j], gs[j]->prec);
+  };
+  if (nact == 0) {
+    finish_failed();
+    return first;
   }
   int rc = batch_reserve(b, count);
-  if (rc) return rc;
+  if (rc) return fail_all(rc);
   for (int j = 0; j < count; j++) {
     Grid* g = gs[j];
-    if ((rc = ensure_records(g, max_iter))) return rc;
+    if ((rc = ensure_records(g, max_iter))) return fail_all(rc);
     fill_host_const(g, edot + 5 * j, dt[j]);
-    HIP_TRY(hipMemcpyAsync(g->dC, g->hC, sizeof(StepConst), hipMemcpyHostToDevice, b->st));
+    if (active[j]) {
+      HIP_TRY(hipMemcpyAsync(g->dC, g->hC, sizeof(StepConst), hipMemcpyHostToDevice, b->st));
+      HIP_TRY(launch_health_reset(g->dhealth, b->st));
+    }
     fill_vox_job(g, b->dls + j, b->hvox + j);
     fill_fft_job(g, &b->dls[j].done, b->hfft + j);
   }
@@ -973,7 +1084,8 @@ static int batch_step(Batch* b, Grid** gs, int count, const double* edot, int ma
   // prologue first (the tuner times the FFT passes on valid Green constants)
   {
     std::map<int, std::vector<int>> pro;
-    for (int j = 0; j < count; j++) pro[gs[j]->prec].push_back(j);
+    for (int j = 0; j < count; j++)
+      if (active[j]) pro[gs[j]->prec].push_back(j);
     int ids_at = 0, off_at = 0;
     std::vector<BLaunch> pl;
     for (auto& kv : pro) {
@@ -988,27 +1100,48 @@ static int batch_step(Batch* b, Grid** gs, int count, const double* edot, int ma
       off_at += L.count + 1;
       pl.push_back(L);
     }
-    for (int j = 0; j < count; j++) b->hids[ids_at + j] = j;
-    HIP_TRY(hipMemcpyAsync(b->dids, b->hids, sizeof(int) * (ids_at + count), hipMemcpyHostToDevice, b->st));
+    int k = 0;
+    for (int j = 0; j < count; j++)
+      if (active[j]) b->hids[ids_at + k++] = j;
+    HIP_TRY(hipMemcpyAsync(b->dids, b->hids, sizeof(int) * (ids_at + nact), hipMemcpyHostToDevice, b->st));
     HIP_TRY(hipMemcpyAsync(b->doff, b->hoff, sizeof(long) * off_at, hipMemcpyHostToDevice, b->st));
     HIP_TRY(hipEventRecord(b->ev[0], b->st));
     for (const BLaunch& L : pl) HIP_TRY(launch_prologue_group(L.prec, vox_group(b, L), L.blocks, b->st));
-    const VoxGroup all{b->dvox, b->dids + ids_at, nullptr, count};
+    const VoxGroup all{b->dvox, b->dids + ids_at, nullptr, nact};
     HIP_TRY(launch_L0_finalize_group(all, b->st));
     HIP_TRY(launch_loop_reset_group(all, b->st));
     for (int j = 0; j < count; j++)
-      HIP_TRY(hipMemcpyAsync(gs[j]->hprol, gs[j]->dprol, sizeof(PrologueRecord), hipMemcpyDeviceToHost, b->st));
+      if (active[j]) {
+        HIP_TRY(hipMemcpyAsync(gs[j]->hprol, gs[j]->dprol, sizeof(PrologueRecord), hipMemcpyDeviceToHost, b->st));
+        HIP_TRY(hipMemcpyAsync(gs[j]->hhealth, gs[j]->dhealth, sizeof(HealthRecord), hipMemcpyDeviceToHost, b->st));
+      }
     HIP_TRY(hipEventRecord(b->ev[1], b->st));
     HIP_TRY(hipStreamSynchronize(b->st));
-    for (int j = 0; j < count; j++)
-      if (gs[j]->hprol->degenerate > 0)
-        return set_error(VPFFT_E_DEGENERATE, "grid %d: %.0f voxels with max resolved shear stress < 1e-10", j,
-                         gs[j]->hprol->degenerate);
+    for (int j = 0; j < count; j++) {
+      if (!active[j]) continue;
+      Grid* g = gs[j];
+      if (g->hprol->degenerate > 0)
+        grid_fail(j, set_error(VPFFT_E_DEGENERATE, "grid %d: %.0f voxels with max resolved shear stress < 1e-10", j,
+                               g->hprol->degenerate));
+      else if (g->hhealth->flag)
+        grid_fail(j, health_error(g, *g->hhealth, "initialization"));
+      else if (!spd5(g->hprol->L0) || !spd5(g->hprol->L0inv))
+        grid_fail(j, set_error(VPFFT_E_NONFINITE, "grid %d: reference medium L0 is not symmetric positive definite",
+                               j));
+      if (status[j]) {
+        active[j] = 0;
+        nact--;
+      }
+    }
   }
-  if ((rc = batch_plan(b, gs, count, dt))) return rc;
+  if (nact == 0) {
+    finish_failed();
+    return first;
+  }
+  if ((rc = batch_plan(b, gs, count, dt, active))) return fail_all(rc);
   // the loop, replayed as a graph of graph_iters iterations
-  std::vector<unsigned long> key_grids(count);
-  for (int j = 0; j < count; j++) key_grids[j] = gs[j]->serial;
+  std::vector<unsigned long> key_grids;
+  for (int j = 0; j < count; j++) key_grids.push_back(active[j] ? gs[j]->serial : 0);
   std::vector<int> key_vals{max_iter, count};
   for (const BLaunch& L : b->launches) {
     key_vals.push_back(L.kind);
@@ -1030,7 +1163,7 @@ static int batch_step(Batch* b, Grid** gs, int count, const double* edot, int ma
       int crc = VPFFT_OK;
       for (int k = 0; k < b->graph_iters && crc == VPFFT_OK; k++) crc = batch_iteration(b, tol, max_iter);
       hipError_t ce = hipStreamEndCapture(b->st, &graph);
-      if (crc) return crc;
+      if (crc) return fail_all(crc);
       HIP_TRY(ce);
       hipError_t ie = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
@@ -1044,29 +1177,36 @@ static int batch_step(Batch* b, Grid** gs, int count, const double* edot, int ma
     if (b->graph_iters > 0) {
       HIP_TRY(hipGraphLaunch(b->gexec, b->st));
     } else if ((rc = batch_iteration(b, tol, max_iter))) {
-      return rc;
+      return fail_all(rc);
     }
     HIP_TRY(hipMemcpyAsync(b->hls, b->dls, sizeof(LoopState) * count, hipMemcpyDeviceToHost, b->st));
     HIP_TRY(hipStreamSynchronize(b->st));
     bool all_done = true;
-    for (int j = 0; j < count; j++) all_done = all_done && b->hls[j].done;
+    for (int j = 0; j < count; j++) all_done = all_done && (!active[j] || b->hls[j].done);
     if (all_done) break;
   }
-  std::vector<int> iters(count);
+  std::vector<int> iters(count, 0);
   for (int j = 0; j < count; j++) {
+    if (!active[j]) continue;
     iters[j] = b->hls[j].iter;
     HIP_TRY(hipMemcpyAsync(gs[j]->hrec, gs[j]->drec, sizeof(IterRecord) * iters[j], hipMemcpyDeviceToHost, b->st));
   }
   HIP_TRY(hipEventRecord(b->ev[2], b->st));
+  // the epilogue kernel returns at entry for a grid whose health flag is set
   for (const BLaunch& L : b->launches)
     if (L.kind == L_EPI) HIP_TRY(launch_epilogue_group(vox_group(b, L), L.blocks, b->st));
   HIP_TRY(hipEventRecord(b->ev[3], b->st));
+  // after the epilogue: the record names the loop's failure, or the
+  // epilogue's own (the record's kernel field tells which)
+  for (int j = 0; j < count; j++)
+    if (active[j])
+      HIP_TRY(hipMemcpyAsync(gs[j]->hhealth, gs[j]->dhealth, sizeof(HealthRecord), hipMemcpyDeviceToHost, b->st));
   HIP_TRY(hipStreamSynchronize(b->st));
   float ms[3] = {0, 0, 0};
   for (int k = 0; k < 3; k++) (void)hipEventElapsedTime(&ms[k], b->ev[k], b->ev[k + 1]);
-  int status = VPFFT_OK;
   for (int j = 0; j < count; j++) {
     Grid* g = gs[j];
+    if (!active[j]) continue;
     g->phi_prec = g->dtil_prec = g->prec;
     const int it = iters[j];
     double nr_total = 0, wave_total = 0;
@@ -1076,7 +1216,12 @@ static int batch_step(Batch* b, Grid** gs, int count, const double* edot, int ma
       wave_total += g->hrec[i].nr_wave_steps;
       if (!degen_at && g->hrec[i].degenerate > 0) degen_at = i + 1;
     }
-    if (dt[j] > 0) {
+    const IterRecord& last = g->hrec[it - 1];
+    bool finite = !g->hhealth->flag;
+    for (int i = 0; i < 5; i++) finite = finite && std::isfinite(last.avg_stress[i]);
+    finite = finite && std::isfinite(last.err_stress) && std::isfinite(last.err_strain_rate);
+    // the epilogue ran (and moved the state on) unless the loop went bad
+    if (dt[j] > 0 && finite && !degen_at) {
       g->crss_uniform = g->crss_uniform && g->hardening_uniform;
       double d[3];
       edot_diag(edot + 5 * j, d);
@@ -1084,16 +1229,14 @@ static int batch_step(Batch* b, Grid** gs, int count, const double* edot, int ma
       g->ly *= (1.0 + d[1] * dt[j]);
       g->lz *= (1.0 + d[2] * dt[j]);
     }
-    const IterRecord& last = g->hrec[it - 1];
-    bool finite = true;
-    for (int i = 0; i < 5; i++) finite = finite && std::isfinite(last.avg_stress[i]);
-    finite = finite && std::isfinite(last.err_stress) && std::isfinite(last.err_strain_rate);
-    const int st = degen_at ? VPFFT_E_DEGENERATE : (finite ? VPFFT_OK : VPFFT_E_NONFINITE);
-    if (st && !status) {
-      status = st;
-      set_error(st, degen_at ? "grid %d: degenerate voxels (iteration %d)" : "grid %d: non-finite result", j,
-                degen_at);
-    }
+    int st = VPFFT_OK;
+    if (degen_at)
+      st = set_error(VPFFT_E_DEGENERATE, "grid %d: degenerate voxels (iteration %d)", j, degen_at);
+    else if (g->hhealth->flag)
+      st = health_error(g, *g->hhealth, "strain step");
+    else if (!finite)
+      st = set_error(VPFFT_E_NONFINITE, "grid %d: non-finite result", j);
+    if (st) grid_fail(j, st);
     if (avg)
       for (int i = 0; i < 5; i++) avg[5 * j + i] = last.avg_stress[i];
     if (infos) {
@@ -1122,7 +1265,8 @@ static int batch_step(Batch* b, Grid** gs, int count, const double* edot, int ma
       info->ms_epilogue = ms[2];
     }
   }
-  return status;
+  finish_failed();
+  return first;
 }
 
 static int batch_create(Batch** out) {

@@ -858,6 +858,102 @@ __device__ __forceinline__ void lu_fullpiv(const double Ac[25], const double bi[
   }
 }
 
+// Inverse of a 5x5 (column-major in and out) by Gauss-Jordan elimination with
+// partial pivoting, the oracle's stand-in for Eigen's inverse()
+// (MaterialGrid.cpp:248, 632), operation for operation: the pivot row is the
+// first maximum of |column k| at or below row k, the pivot row is scaled by
+// the reciprocal of the pivot, every other row with a nonzero multiplier is
+// reduced.  Columns of the left block at or left of the pivot are not read
+// again, so they are not kept.  Returns false on an exactly zero pivot
+// (out is then not meaningful).
+__device__ __forceinline__ bool inverse_gj5(const double Ac[25], double out[25]) {
+#pragma clang fp contract(off)
+  double L[5][5], Q[5][5];  // left block, right block (row i, column j)
+#pragma unroll
+  for (int i = 0; i < 5; i++)
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      L[i][j] = Ac[j * 5 + i];
+      Q[i][j] = i == j ? 1.0 : 0.0;
+    }
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    int p = k;
+    double best = fabs(L[k][k]);
+#pragma unroll
+    for (int i = k + 1; i < 5; i++) {
+      const double v = fabs(L[i][k]);
+      const bool gt = v > best;
+      best = gt ? v : best;
+      p = gt ? i : p;
+    }
+    ok = ok && best != 0.0;
+#pragma unroll
+    for (int i = k + 1; i < 5; i++) {
+      const bool sw = p == i;
+#pragma unroll
+      for (int j = k; j < 5; j++) {
+        const double t = L[k][j];
+        L[k][j] = sw ? L[i][j] : t;
+        L[i][j] = sw ? t : L[i][j];
+      }
+#pragma unroll
+      for (int j = 0; j < 5; j++) {
+        const double t = Q[k][j];
+        Q[k][j] = sw ? Q[i][j] : t;
+        Q[i][j] = sw ? t : Q[i][j];
+      }
+    }
+    const double inv = 1.0 / L[k][k];
+#pragma unroll
+    for (int j = k + 1; j < 5; j++) L[k][j] *= inv;
+#pragma unroll
+    for (int j = 0; j < 5; j++) Q[k][j] *= inv;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      if (i == k) continue;
+      const double f = L[i][k];
+      const bool nz = f != 0.0;
+#pragma unroll
+      for (int j = k + 1; j < 5; j++) L[i][j] = nz ? L[i][j] - f * L[k][j] : L[i][j];
+#pragma unroll
+      for (int j = 0; j < 5; j++) Q[i][j] = nz ? Q[i][j] - f * Q[k][j] : Q[i][j];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 5; i++)
+#pragma unroll
+    for (int j = 0; j < 5; j++) out[j * 5 + i] = Q[i][j];
+  return ok;
+}
+
+// The voxel's secant modulus M = sum_k m_k m_k^T gdot0_k |r_k|^(n_k - 1) / tau_k
+// of BuildLinearRefMediumStiffness (MaterialGrid.cpp:228-243) at the Taylor
+// stress s, column-major, in the oracle's operation order (quotients by
+// tau, the correctly rounded power).  Exactly symmetric: m_i m_j == m_j m_i.
+__device__ __forceinline__ void secant_modulus(const double m[kSlips][5], const double tau[kSlips], const double s[5],
+                                               const StepConst& C, double M[25]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int e = 0; e < 25; e++) M[e] = 0.0;
+#pragma unroll
+  for (int k = 0; k < kSlips; k++) {
+    const double rt = 1.0 / tau[k];
+    const double rss = div_cr(dot5(m[k], s), tau[k], rt);
+    const int n = C.uniform_n ? C.nexp[0] : C.nexp[k];
+    const double g = (n - 1 > 0) ? C.gdot0[k] * ipow_cr(fabs(rss), n - 1) : C.gdot0[k];
+#pragma unroll
+    for (int j = 0; j < 5; j++)
+#pragma unroll
+      for (int i = j; i < 5; i++) M[j * 5 + i] += div_cr(m[k][i] * m[k][j] * g, tau[k], rt);
+  }
+#pragma unroll
+  for (int j = 0; j < 5; j++)
+#pragma unroll
+    for (int i = 0; i < j; i++) M[j * 5 + i] = M[i * 5 + j];
+}
+
 // Solvers::SolveConstitutiveEquations (Solvers.cpp:19-101), operation for operation.
 __device__ __forceinline__ void taylor_nr(const double init[5], const double m[kSlips][5], const double tau[kSlips],
                                           const StepConst& C, double eps, int max_it, double out[5],

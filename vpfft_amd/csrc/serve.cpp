@@ -61,12 +61,31 @@ extern "C" int vpfft_serve(const vpfft_serve_config* cfg, const vpfft_request* r
     return set_error(VPFFT_E_INVALID, "precision must be VPFFT_FP64 or VPFFT_MIXED");
   if (cfg->max_active < 1 || cfg->hbm_budget_bytes == 0 || cfg->max_iter < 1 || !(cfg->tol >= 0))
     return set_error(VPFFT_E_INVALID, "bad serving configuration");
+  std::memset(res, 0, sizeof(vpfft_request_result) * (size_t)count);
+  // a request with bad fields is refused alone (its status says why); the
+  // others are served
+  std::vector<char> refused(count, 0);
   for (int i = 0; i < count; i++) {
     const vpfft_request& q = reqs[i];
-    if (q.edge < 2 || q.grains < 1 || q.num_increments < 1 || !q.edot || !std::isfinite(q.arrival_s))
-      return set_error(VPFFT_E_INVALID, "request %d: bad fields", i);
+    int st = VPFFT_OK;
+    if (q.edge < 2 || q.grains < 1 || q.num_increments < 1 || !q.edot || !std::isfinite(q.arrival_s) ||
+        !std::isfinite(q.dt))
+      st = set_error(VPFFT_E_INVALID, "request %d: bad fields", i);
+    for (int k = 0; !st && k < q.num_increments; k++) {
+      double ee = 0;
+      bool fin = true;
+      for (int c = 0; c < 5; c++) {
+        fin = fin && std::isfinite(q.edot[5 * k + c]);
+        ee += q.edot[5 * k + c] * q.edot[5 * k + c];
+      }
+      if (!fin) st = set_error(VPFFT_E_NONFINITE, "request %d: non-finite strain rate in increment %d", i, k);
+      else if (!(ee > 0)) st = set_error(VPFFT_E_DEGENERATE, "request %d: zero strain rate in increment %d", i, k);
+    }
+    if (st) {
+      res[i].status = st;
+      refused[i] = 1;
+    }
   }
-  std::memset(res, 0, sizeof(vpfft_request_result) * (size_t)count);
   const bool fallback = cfg->fp64_fallback && cfg->precision == VPFFT_MIXED;
   std::vector<int> order(count);
   for (int i = 0; i < count; i++) order[i] = i;
@@ -103,6 +122,10 @@ extern "C" int vpfft_serve(const vpfft_serve_config* cfg, const vpfft_request* r
     while (next_pending < order.size()) {
       const int qi = order[next_pending];
       const vpfft_request& q = reqs[qi];
+      if (refused[qi]) {
+        next_pending++;
+        continue;
+      }
       if (q.arrival_s > seconds_since(t0) || (int)active.size() >= cfg->max_active) break;
       const vpfft_grid_desc desc = {q.edge, q.edge, q.edge, cfg->precision};
       uint64_t bytes = 0;
@@ -170,11 +193,13 @@ extern "C" int vpfft_serve(const vpfft_serve_config* cfg, const vpfft_request* r
       res[active[j].req].iterations += infos[j].iterations;
       voxel_iters += active[j].voxels * infos[j].iterations;
     }
-    // mixed steps that ended unconverged and stalled: rerun them in FP64 from the saved state
+    // mixed steps that ended unconverged and stalled, or met a non-finite
+    // value: rerun them in FP64 from the saved state
     if (fallback) {
       std::vector<int> redo;
       for (int j = 0; j < m; j++)
-        if (infos[j].status == VPFFT_OK && !infos[j].converged && infos[j].err_ratio_5 >= cfg->stall_ratio)
+        if ((infos[j].status == VPFFT_OK && !infos[j].converged && infos[j].err_ratio_5 >= cfg->stall_ratio) ||
+            infos[j].status == VPFFT_E_NONFINITE)
           redo.push_back(j);
       if (!redo.empty()) {
         std::vector<void*> rg;
@@ -198,7 +223,8 @@ extern "C" int vpfft_serve(const vpfft_serve_config* cfg, const vpfft_request* r
           infos[j] = rinf[k];
           res[active[j].req].iterations += rinf[k].iterations;
           voxel_iters += active[j].voxels * rinf[k].iterations;
-          if (rrc && infos[j].status == VPFFT_OK) infos[j].status = rrc;
+          if (rrc && rrc != VPFFT_E_DEGENERATE && rrc != VPFFT_E_NONFINITE && infos[j].status == VPFFT_OK)
+            infos[j].status = rrc;
           res[active[j].req].fp64_reruns++;
           reruns_total++;
         }

@@ -52,26 +52,58 @@ struct LoopState {
 
 struct PrologueRecord {
   double L0[25];
+  double L0inv[25];     // column-major
   double degenerate;
   double taylor_steps;
 };
 
+// The first non-finite value a kernel of the current strain step produced or
+// read (flag 0: none).  Written once per step by whichever lane gets there
+// first; reset at the start of every step.  The epilogue does nothing once it
+// is set, so a failed step leaves the orientation and CRSS as they were.
+struct HealthRecord {
+  int flag;
+  int kernel;   // HealthKernel
+  int field;    // HealthField
+  int iter;     // fixed-point iteration (1-based) for the loop kernels, 0 outside
+  long voxel;   // x*Ny*Nz + y*Nz + z
+  double value;
+};
+enum HealthKernel { HK_NONE = 0, HK_PROLOGUE = 1, HK_L0 = 2, HK_NR = 3, HK_EPILOGUE = 4 };
+enum HealthField {
+  HF_NONE = 0,
+  HF_TAYLOR_STRESS = 1,    // the Taylor initialisation's stress
+  HF_SECANT_INVERSE = 2,   // the inverse of the voxel's secant modulus (singular or non-finite)
+  HF_L0 = 3,               // the reference medium or its inverse
+  HF_DTILDE = 4,           // the strain-rate variation the FFT passes produced
+  HF_STRESS = 5,           // the constrained Newton solve's stress
+  HF_STRAIN_RATE = 6,      // the constitutive strain rate of that stress
+  HF_LAMBDA = 7,           // the updated Lagrange multiplier
+  HF_ORIENTATION = 8,      // the epilogue's orientation update
+  HF_CRSS = 9              // the epilogue's hardening
+};
+const char* health_kernel_name(int k);
+const char* health_field_name(int f);
+
 hipError_t launch_prologue(int prec, long n, const double* R, const double* crss, const StepConst* C, void* field,
-                           double* sigma, double* lambda, double* partial, hipStream_t st);
+                           double* sigma, double* lambda, double* partial, HealthRecord* health, hipStream_t st);
 hipError_t launch_L0_finalize(long n, int nblocks, const double* partial, StepConst* C, GreenConst<double>* g64,
-                              GreenConst<float>* g32, PrologueRecord* rec, hipStream_t st);
+                              GreenConst<float>* g32, PrologueRecord* rec, HealthRecord* health, hipStream_t st);
+hipError_t launch_health_reset(HealthRecord* health, hipStream_t st);
 // derive L0^-1 and the Green constants from C->L0 (already set)
 hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32, hipStream_t st);
 // waves: 1 or 2 waves per SIMD (register budget 512 or 256 per lane);
 // uniform_tau: every slip system of a voxel has the same CRSS (crss[0][v] is read)
+// ls: the loop state (kernels return at entry once ls->done is set; may be
+// null outside a loop); health: where the first non-finite value is recorded
 hipError_t launch_nr(int prec, int nexp_uniform, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
                      double* lambda, const double* R, const double* crss, const StepConst* C, double* partial,
-                     const int* done, hipStream_t st);
+                     const LoopState* ls, HealthRecord* health, hipStream_t st);
 hipError_t launch_iter_finalize(long n, const double* partial, const StepConst* C, double tol,
                                 int max_iter, LoopState* ls, IterRecord* rec, hipStream_t st);
 hipError_t launch_loop_reset(LoopState* ls, hipStream_t st);
 hipError_t launch_epilogue(long n, double* R, double* crss, double* acc, const double* sigma, const StepConst* C,
-                           hipStream_t st);
+                           HealthRecord* health, hipStream_t st);
 hipError_t launch_aos_to_soa(const double* aos, double* soa, long count, int width, long soa_stride, long offset,
                              hipStream_t st);
 hipError_t launch_soa_to_aos(const double* soa, double* aos, long count, int width, long soa_stride, long offset,
@@ -104,6 +136,7 @@ struct VoxJob {
   LoopState* ls;
   IterRecord* rec;
   PrologueRecord* prol;
+  HealthRecord* health;
 };
 // A batched launch: blocks [off[m], off[m+1]) belong to jobs[ids[m]]; the
 // one-block-per-job kernels use ids[0..count).

@@ -25,6 +25,47 @@ namespace vpfft {
 
 template <typename T> __device__ __forceinline__ double ld(const T* p) { return (double)*p; }
 
+const char* health_kernel_name(int k) {
+  switch (k) {
+    case HK_PROLOGUE: return "prologue";
+    case HK_L0: return "reference medium";
+    case HK_NR: return "Newton kernel";
+    case HK_EPILOGUE: return "epilogue";
+    default: return "none";
+  }
+}
+const char* health_field_name(int f) {
+  switch (f) {
+    case HF_TAYLOR_STRESS: return "Taylor stress";
+    case HF_SECANT_INVERSE: return "inverse secant modulus";
+    case HF_L0: return "L0 or its inverse";
+    case HF_DTILDE: return "strain-rate variation (FFT output)";
+    case HF_STRESS: return "stress";
+    case HF_STRAIN_RATE: return "constitutive strain rate";
+    case HF_LAMBDA: return "Lagrange multiplier";
+    case HF_ORIENTATION: return "orientation";
+    case HF_CRSS: return "CRSS";
+    default: return "none";
+  }
+}
+
+// Record the first non-finite value of the step (cold path: one lane of the
+// grid wins the flag and fills the record).
+__device__ __forceinline__ void report_health(HealthRecord* h, int kernel, int field, int iter, long voxel,
+                                              double value) {
+  if (h && atomicCAS(&h->flag, 0, 1) == 0) {
+    h->kernel = kernel;
+    h->field = field;
+    h->iter = iter;
+    h->voxel = voxel;
+    h->value = value;
+  }
+}
+
+// the sum of five values is finite iff all are (barring overflow of the sum,
+// which is reported as non-finite too)
+__device__ __forceinline__ double sum5(const double a[5]) { return a[0] + a[1] + a[2] + a[3] + a[4]; }
+
 // Reduce W values over the block in a fixed order; thread 0 gets the result.
 // W sums then M maxima go to out[0..W+M).
 template <int W, int M = 1>
@@ -87,21 +128,6 @@ __device__ __forceinline__ void load_inv_tau(const double* __restrict__ crss, lo
   for (int k = 0; k < kSlips; k++) it[k] = 1.0 / crss[k * n + v];
 }
 
-// Inverse of an SPD 5x5 (packed lower) by LDL^T solves against unit vectors.
-__device__ __forceinline__ void spd_inverse5(const double A[15], double Ainv[15]) {
-#pragma unroll
-  for (int c = 0; c < 5; c++) {
-    double Acopy[15];
-#pragma unroll
-    for (int e = 0; e < 15; e++) Acopy[e] = A[e];
-    double b[5] = {0, 0, 0, 0, 0}, x[5];
-    b[c] = 1.0;
-    ldlt_solve5(Acopy, b, x);
-#pragma unroll
-    for (int i = c; i < 5; i++) Ainv[sidx(i, c)] = x[i];
-  }
-}
-
 // Voxel of thread tid in block blk of a one-voxel-per-thread launch.  When Ny % 4 == 0 and Nz % 16 == 0
 // each wave takes a 4 (y) x 16 (z) tile of one x-plane instead of 64
 // consecutive z: 16-lane runs still read whole 128-byte lines, and a compact
@@ -139,89 +165,132 @@ __device__ __forceinline__ long cube_voxel(long blk, int tid, int nx, int ny, in
 // InitializeComputation + the secant moduli of BuildLinearRefMediumStiffness
 // for voxel v; adds M_loc^-1 (15), the degenerate count (slot 15) and the
 // Taylor Newton steps (slot 16) to acc.
+// InitializeComputation for voxel v (MaterialGrid.cpp:153-210): sigma,
+// lambda and the polarization get the Taylor stress; the degenerate count and
+// the Taylor Newton steps go to acc[0], acc[1].
 template <typename T>
 __device__ __forceinline__ void prologue_voxel(long v, long n, const double* __restrict__ R,
                                                const double* __restrict__ crss, const StepConst& C,
                                                T* __restrict__ field, double* __restrict__ sigma,
-                                               double* __restrict__ lambda, double (&acc)[17]) {
-  {
-    double r[9], tau[kSlips], it[kSlips], m[kSlips][5];
-    load_R(R, n, v, r);
+                                               double* __restrict__ lambda, double (&acc)[2],
+                                               HealthRecord* __restrict__ health) {
+  double r[9], tau[kSlips], m[kSlips][5];
+  load_R(R, n, v, r);
 #pragma unroll
-    for (int k = 0; k < kSlips; k++) {
-      tau[k] = crss[k * n + v];
-      it[k] = 1.0 / tau[k];
-    }
-    // bit-faithful: Schmid tensors, Sachs guess (Solvers.cpp:230-243), max RSS
-    // rescale (MaterialGrid.cpp:175-190), Taylor NR (Solvers.cpp:19-101)
-    faithful::schmid(r, m);
-    double guess[5];
-    faithful::sachs(m, tau, C.edot, guess);
-    const double mrss = faithful::max_rss(guess, m, tau);
-    if (mrss > 1.1) {
+  for (int k = 0; k < kSlips; k++) tau[k] = crss[k * n + v];
+  // bit-faithful: Schmid tensors, Sachs guess (Solvers.cpp:230-243), max RSS
+  // rescale (MaterialGrid.cpp:175-190), Taylor NR (Solvers.cpp:19-101)
+  faithful::schmid(r, m);
+  double guess[5];
+  faithful::sachs(m, tau, C.edot, guess);
+  const double mrss = faithful::max_rss(guess, m, tau);
+  if (mrss > 1.1) {
 #pragma unroll
-      for (int i = 0; i < 5; i++) guess[i] /= mrss;
-    }
-    if (!(mrss >= 1e-10)) acc[15] += 1.0;  // degenerate (the reference exits here)
-    double s[5];
-    int tsteps = 0;
-    faithful::taylor_nr(guess, m, tau, C, 5e-4, 1000, s, &tsteps);
-    acc[16] += (double)tsteps;
+    for (int i = 0; i < 5; i++) guess[i] /= mrss;
+  }
+  if (!(mrss >= 1e-10)) acc[0] += 1.0;  // degenerate (the reference exits here)
+  double s[5];
+  int tsteps = 0;
+  faithful::taylor_nr(guess, m, tau, C, 5e-4, 1000, s, &tsteps);
+  acc[1] += (double)tsteps;
 #pragma unroll
-    for (int i = 0; i < 5; i++) {
-      sigma[i * n + v] = s[i];
-      lambda[i * n + v] = s[i];
-      field[i * n + v] = (T)s[i];
-    }
-    // secant modulus M = sum m m^T g / tau (MaterialGrid.cpp:228-243), inverted
-    double M[15];
+  for (int i = 0; i < 5; i++) {
+    sigma[i * n + v] = s[i];
+    lambda[i * n + v] = s[i];
+    field[i * n + v] = (T)s[i];
+  }
+  if (!__builtin_isfinite(sum5(s))) report_health(health, HK_PROLOGUE, HF_TAYLOR_STRESS, 0, v, sum5(s));
+}
+
+// The per-voxel half of BuildLinearRefMediumStiffness (MaterialGrid.cpp:
+// 219-250) for voxel v at its Taylor stress: the secant modulus and its
+// inverse as the oracle computes them (the inverses of nearly singular moduli
+// carry most of L0's weight, so nothing is cut off or regularised); the
+// inverse's lower triangle is added to acc.  A separate kernel from the
+// Taylor solve, which needs most of the register file.
+__device__ __forceinline__ void secant_voxel(long v, long n, const double* __restrict__ R,
+                                             const double* __restrict__ crss, const double* __restrict__ sigma,
+                                             const StepConst& C, double (&acc)[15],
+                                             HealthRecord* __restrict__ health) {
+  double r[9], tau[kSlips], m[kSlips][5], s[5];
+  load_R(R, n, v, r);
 #pragma unroll
-    for (int e = 0; e < 15; e++) M[e] = 0.0;
+  for (int k = 0; k < kSlips; k++) tau[k] = crss[k * n + v];
 #pragma unroll
-    for (int k = 0; k < kSlips; k++) {
-      const double rs = dot5(m[k], s) * it[k];
-      const double c = gamma_nm1_taylor(C, k, fabs(rs)) * it[k];
+  for (int i = 0; i < 5; i++) s[i] = sigma[i * n + v];
+  faithful::schmid(r, m);
+  double M[25], Mi[25];
+  faithful::secant_modulus(m, tau, s, C, M);
+  const bool ok = faithful::inverse_gj5(M, Mi);
+  double chk = 0.0;
 #pragma unroll
-      for (int i = 0; i < 5; i++)
+  for (int e = 0; e < 25; e++) chk += Mi[e];
+  if (!ok || !__builtin_isfinite(chk)) {
+    report_health(health, HK_PROLOGUE, HF_SECANT_INVERSE, 0, v, ok ? chk : 0.0);
+  } else {
 #pragma unroll
-        for (int j = 0; j <= i; j++) M[sidx(i, j)] += c * m[k][i] * m[k][j];
-    }
-    double Mi[15];
-    spd_inverse5(M, Mi);
+    for (int i = 0; i < 5; i++)
 #pragma unroll
-    for (int e = 0; e < 15; e++) acc[e] += Mi[e];
+      for (int j = 0; j <= i; j++) acc[sidx(i, j)] += Mi[j * 5 + i];
   }
 }
 
+// Prologue partial rows are 18 wide: the Taylor kernel's block b fills slots
+// 15 (degenerate voxels) and 16 (Taylor Newton steps) of row b, the secant
+// kernel's block b slots 0..14 (inverse secant moduli, packed lower) of row
+// nb + b, where nb is the number of blocks of either launch.
 template <typename T>
 __global__ void __launch_bounds__(kPointThreads) k_prologue(long n, const double* __restrict__ R,
                                                             const double* __restrict__ crss,
                                                             const StepConst* __restrict__ Cp, T* __restrict__ field,
                                                             double* __restrict__ sigma, double* __restrict__ lambda,
-                                                            double* __restrict__ partial) {
+                                                            double* __restrict__ partial,
+                                                            HealthRecord* __restrict__ health) {
   const StepConst& C = *Cp;
-  double acc[17];
-#pragma unroll
-  for (int w = 0; w < 17; w++) acc[w] = 0.0;
+  double acc[2] = {0.0, 0.0};
   double mx[1] = {0.0};
   // linear index i walks the blocks of cube_voxel (a permutation of the voxels)
   for (long i = (long)blockIdx.x * kPointThreads + threadIdx.x; i < (long)pt_blocks(n) * kPointThreads;
        i += (long)gridDim.x * kPointThreads) {
     const long v = cube_voxel(i / kPointThreads, threadIdx.x, C.nx, C.ny, C.nz);
-    if (v < n) prologue_voxel<T>(v, n, R, crss, C, field, sigma, lambda, acc);
+    if (v < n) prologue_voxel<T>(v, n, R, crss, C, field, sigma, lambda, acc, health);
   }
-  block_sum<17>(acc, mx, partial + (long)blockIdx.x * 18);
+  block_sum<2>(acc, mx, partial + (long)blockIdx.x * 18 + 15);
 }
 
-// L0 = (1/N) sum of the per-voxel inverses; derived constants.
-__device__ void derive_constants(StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32) {
-  // L0^-1
-  double A[15], Ai[15];
+__global__ void __launch_bounds__(kPointThreads) k_secant(long n, const double* __restrict__ R,
+                                                          const double* __restrict__ crss,
+                                                          const double* __restrict__ sigma,
+                                                          const StepConst* __restrict__ Cp,
+                                                          double* __restrict__ partial,
+                                                          HealthRecord* __restrict__ health) {
+  const StepConst& C = *Cp;
+  double acc[15];
+#pragma unroll
+  for (int w = 0; w < 15; w++) acc[w] = 0.0;
+  double mx[1] = {0.0};
+  for (long i = (long)blockIdx.x * kPointThreads + threadIdx.x; i < (long)pt_blocks(n) * kPointThreads;
+       i += (long)gridDim.x * kPointThreads) {
+    const long v = cube_voxel(i / kPointThreads, threadIdx.x, C.nx, C.ny, C.nz);
+    if (v < n) secant_voxel(v, n, R, crss, sigma, C, acc, health);
+  }
+  block_sum<15>(acc, mx, partial + ((long)gridDim.x + blockIdx.x) * 18);
+}
+
+// Constants derived from L0 (already in C): L0^-1 and the Green-operator
+// coefficients.  Returns false when L0 is singular or L0^-1 not finite.
+__device__ bool derive_constants(StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32) {
+  // L0^-1 by the oracle's Gauss-Jordan (the special-plane operator is
+  // (-L0)^-1, MaterialGrid.cpp:628-632), symmetrised
+  double Li[25];
+  bool ok = faithful::inverse_gj5(C->L0, Li);
   for (int i = 0; i < 5; i++)
-    for (int j = 0; j <= i; j++) A[sidx(i, j)] = C->L0[j * 5 + i];
-  spd_inverse5(A, Ai);
-  for (int i = 0; i < 5; i++)
-    for (int j = 0; j < 5; j++) C->L0inv[j * 5 + i] = Ai[sidx(i, j)];
+    for (int j = 0; j <= i; j++) {
+      const double v = Li[j * 5 + i];
+      ok = ok && __builtin_isfinite(v);
+      C->L0inv[j * 5 + i] = v;
+      C->L0inv[i * 5 + j] = v;
+    }
   // L_ijkl = sum_ab L0(a,b) B_a,ij B_b,kl ; K_ik = sum_jl L_ijkl k_j k_l
   const double s2 = 0.70710678118654752440, s6 = 0.40824829046386301637;
   double Bm[5][3][3];
@@ -259,17 +328,20 @@ __device__ void derive_constants(StepConst* C, GreenConst<double>* g64, GreenCon
   g64->nx = g32->nx = C->nx;
   g64->ny = g32->ny = C->ny;
   g64->nz = g32->nz = C->nz;
+  return ok;
 }
 
 // L0 from nblocks prologue partial rows (18 wide); one block.
 __device__ __forceinline__ void l0_finalize_body(long n, int nblocks, const double* __restrict__ partial,
                                                  StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32,
-                                                 PrologueRecord* rec) {
+                                                 PrologueRecord* rec, HealthRecord* health) {
   __shared__ double sh[256][18];
   double acc[18];
   for (int w = 0; w < 18; w++) acc[w] = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
-    for (int w = 0; w < 18; w++) acc[w] += partial[(long)b * 18 + w];
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    for (int w = 0; w < 15; w++) acc[w] += partial[((long)nblocks + b) * 18 + w];
+    for (int w = 15; w < 17; w++) acc[w] += partial[(long)b * 18 + w];
+  }
   for (int w = 0; w < 18; w++) sh[threadIdx.x][w] = acc[w];
   __syncthreads();
   for (int stride = 128; stride > 0; stride >>= 1) {
@@ -286,8 +358,13 @@ __device__ __forceinline__ void l0_finalize_body(long n, int nblocks, const doub
         C->L0[j * 5 + i] = v;
         C->L0[i * 5 + j] = v;
       }
-    derive_constants(C, g64, g32);
-    for (int e = 0; e < 25; e++) rec->L0[e] = C->L0[e];
+    bool ok = derive_constants(C, g64, g32);
+    for (int e = 0; e < 25; e++) {
+      ok = ok && __builtin_isfinite(C->L0[e]);
+      rec->L0[e] = C->L0[e];
+      rec->L0inv[e] = C->L0inv[e];
+    }
+    if (!ok) report_health(health, HK_L0, HF_L0, 0, -1, C->L0[0]);
     rec->degenerate = sh[0][15];
     rec->taylor_steps = sh[0][16];
   }
@@ -296,8 +373,24 @@ __device__ __forceinline__ void l0_finalize_body(long n, int nblocks, const doub
 
 __global__ void __launch_bounds__(256) k_L0_finalize(long n, int nblocks, const double* __restrict__ partial,
                                                      StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32,
-                                                     PrologueRecord* rec) {
-  l0_finalize_body(n, nblocks, partial, C, g64, g32, rec);
+                                                     PrologueRecord* rec, HealthRecord* health) {
+  l0_finalize_body(n, nblocks, partial, C, g64, g32, rec, health);
+}
+
+__global__ void k_health_reset(HealthRecord* h) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    h->flag = 0;
+    h->kernel = HK_NONE;
+    h->field = HF_NONE;
+    h->iter = 0;
+    h->voxel = -1;
+    h->value = 0.0;
+  }
+}
+
+hipError_t launch_health_reset(HealthRecord* h, hipStream_t st) {
+  hipLaunchKernelGGL(k_health_reset, dim3(1), dim3(64), 0, st, h);
+  return hipGetLastError();
 }
 
 __global__ void k_loop_reset(LoopState* ls) {
@@ -313,7 +406,7 @@ hipError_t launch_loop_reset(LoopState* ls, hipStream_t st) {
 }
 
 __global__ void k_derive(StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) derive_constants(C, g64, g32);
+  if (threadIdx.x == 0 && blockIdx.x == 0) (void)derive_constants(C, g64, g32);
 }
 
 // values alike in every lane of the block, moved to scalar registers
@@ -364,6 +457,7 @@ __device__ __forceinline__ VoxJob vox_job(const VoxJob* J) {
   r.ls = vuni_global(J->ls);
   r.rec = vuni_global(J->rec);
   r.prol = vuni_global(J->prol);
+  r.health = vuni_global(J->health);
   return r;
 }
 
@@ -381,16 +475,21 @@ template <typename T, int NE, bool UNIFORM_TAU, bool BATCHED>
 __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu(kNrWaves, kNrWaves)))
 k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restrict__ sigma_,
      double* __restrict__ lambda_, const double* __restrict__ R_, const double* __restrict__ crss_,
-     const StepConst* __restrict__ Cp_, double* __restrict__ partial_, const int* __restrict__ done, VoxGroup grp) {
+     const StepConst* __restrict__ Cp_, double* __restrict__ partial_, const LoopState* __restrict__ ls_,
+     HealthRecord* __restrict__ health_, VoxGroup grp) {
   long n, blk;
   const T* dtil;
   T* phi;
   double *sigma, *lambda, *partial;
   const double *R, *crss;
   const StepConst* Cp;
+  const LoopState* ls;
+  HealthRecord* health;
   if constexpr (BATCHED) {
     const VoxJob J = vox_job(grp.jobs + vox_member(grp, blockIdx.x, &blk));
     if (J.ls->done) return;
+    ls = J.ls;
+    health = J.health;
     n = J.n;
     dtil = (const T*)J.dtil;
     phi = (T*)J.phi;
@@ -401,7 +500,9 @@ k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restr
     Cp = J.C;
     partial = J.partial;
   } else {
-    if (done && *done) return;
+    if (ls_ && ls_->done) return;
+    ls = ls_;
+    health = health_;
     n = n_;
     dtil = dtil_;
     phi = phi_;
@@ -508,6 +609,16 @@ k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restr
     acc[12] += (double)steps;
     mx[0] = fmax(mx[0], err);
     mx[1] = fmax(mx[1], (double)steps);
+    // the first non-finite value of the step: the FFT's output, then what
+    // this kernel made of it
+    const double fd = sum5(d), fs = sum5(s), fe = sum5(e), fl = sum5(lam);
+    if (__builtin_expect(!__builtin_isfinite(fd + fs + fe + fl), 0)) {
+      const int f = !__builtin_isfinite(fd) ? HF_DTILDE
+                    : !__builtin_isfinite(fs) ? HF_STRESS
+                    : !__builtin_isfinite(fe) ? HF_STRAIN_RATE : HF_LAMBDA;
+      const double val = f == HF_DTILDE ? fd : f == HF_STRESS ? fs : f == HF_STRAIN_RATE ? fe : fl;
+      report_health(health, HK_NR, f, ls ? ls->iter + 1 : 0, v, val);
+    }
   }
   // lane efficiency of the Newton loop: a wave runs as many steps as its
   // slowest lane
@@ -608,7 +719,7 @@ __global__ void __launch_bounds__(256) k_iter_finalize(long n, int nblocks, cons
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void epilogue_voxel(long v, long n, double* __restrict__ R, double* __restrict__ crss,
                                                double* __restrict__ acc_shear, const double* __restrict__ sigma,
-                                               const StepConst& C) {
+                                               const StepConst& C, HealthRecord* __restrict__ health) {
   {
     double r[9], tau[kSlips], it[kSlips], m[kSlips][5], s[5];
     load_R(R, n, v, r);
@@ -630,11 +741,14 @@ __device__ __forceinline__ void epilogue_voxel(long v, long n, double* __restric
     }
     const double dt = C.dt;
     const double Wm[9] = {0.0, w01, w02, -w01, 0.0, w12, -w02, -w12, 0.0};
+    double rsum = 0.0;
 #pragma unroll
     for (int e = 0; e < 9; e++) {
       r[e] += Wm[e] * dt;
       R[e * n + v] = r[e];
+      rsum += r[e];
     }
+    if (!__builtin_isfinite(rsum)) report_health(health, HK_EPILOGUE, HF_ORIENTATION, 0, v, rsum);
     // hardening with the updated Schmid tensors, Solvers.cpp:314-373
     schmid_from_R(r, m);
     double g[kSlips], local = 0.0;
@@ -662,7 +776,9 @@ __device__ __forceinline__ void epilogue_voxel(long v, long n, double* __restric
                  C.theta1[i] / fr * e_ini * (e_del * ((A + local) * fr + 1.0) - (A * fr + 1.0));
         }
       }
-      crss[i * n + v] = tau[i] + dtau * voce / local;
+      const double t = tau[i] + dtau * voce / local;
+      crss[i * n + v] = t;
+      if (!__builtin_isfinite(t)) report_health(health, HK_EPILOGUE, HF_CRSS, 0, v, t);
     }
     acc_shear[v] = A + local;
   }
@@ -671,9 +787,12 @@ __device__ __forceinline__ void epilogue_voxel(long v, long n, double* __restric
 __global__ void __launch_bounds__(kPointThreads) k_epilogue(long n, double* __restrict__ R, double* __restrict__ crss,
                                                             double* __restrict__ acc_shear,
                                                             const double* __restrict__ sigma,
-                                                            const StepConst* __restrict__ Cp) {
+                                                            const StepConst* __restrict__ Cp,
+                                                            HealthRecord* __restrict__ health) {
+  // a step that met a non-finite value leaves the orientation and CRSS alone
+  if (health->flag) return;
   for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x)
-    epilogue_voxel(v, n, R, crss, acc_shear, sigma, *Cp);
+    epilogue_voxel(v, n, R, crss, acc_shear, sigma, *Cp, health);
 }
 
 // ---------------------------------------------------------------------------
@@ -685,19 +804,30 @@ template <typename T>
 __global__ void __launch_bounds__(kPointThreads) k_prologue_b(VoxGroup grp) {
   long local;
   const VoxJob J = vox_job(grp.jobs + vox_member(grp, blockIdx.x, &local));
-  double acc[17];
-#pragma unroll
-  for (int w = 0; w < 17; w++) acc[w] = 0.0;
+  double acc[2] = {0.0, 0.0};
   double mx[1] = {0.0};
   const StepConst& C = *as_const_mem(J.C);
   const long v = cube_voxel(local, threadIdx.x, C.nx, C.ny, C.nz);
-  if (v < J.n) prologue_voxel<T>(v, J.n, J.R, J.crss, C, (T*)J.phi, J.sigma, J.lambda, acc);
-  block_sum<17>(acc, mx, J.partial + local * 18);
+  if (v < J.n) prologue_voxel<T>(v, J.n, J.R, J.crss, C, (T*)J.phi, J.sigma, J.lambda, acc, J.health);
+  block_sum<2>(acc, mx, J.partial + local * 18 + 15);
+}
+
+__global__ void __launch_bounds__(kPointThreads) k_secant_b(VoxGroup grp) {
+  long local;
+  const VoxJob J = vox_job(grp.jobs + vox_member(grp, blockIdx.x, &local));
+  double acc[15];
+#pragma unroll
+  for (int w = 0; w < 15; w++) acc[w] = 0.0;
+  double mx[1] = {0.0};
+  const StepConst& C = *as_const_mem(J.C);
+  const long v = cube_voxel(local, threadIdx.x, C.nx, C.ny, C.nz);
+  if (v < J.n) secant_voxel(v, J.n, J.R, J.crss, J.sigma, C, acc, J.health);
+  block_sum<15>(acc, mx, J.partial + ((long)pt_blocks(J.n) + local) * 18);
 }
 
 __global__ void __launch_bounds__(256) k_L0_finalize_b(VoxGroup grp) {
   const VoxJob J = vox_job(grp.jobs + vuni(grp.ids[blockIdx.x]));
-  l0_finalize_body(J.n, pt_blocks(J.n), J.partial, J.C, J.g64, J.g32, J.prol);
+  l0_finalize_body(J.n, pt_blocks(J.n), J.partial, J.C, J.g64, J.g32, J.prol, J.health);
 }
 
 __global__ void k_loop_reset_b(VoxGroup grp) {
@@ -717,8 +847,9 @@ __global__ void __launch_bounds__(256) k_iter_finalize_b(VoxGroup grp, double to
 __global__ void __launch_bounds__(kPointThreads) k_epilogue_b(VoxGroup grp) {
   long local;
   const VoxJob J = vox_job(grp.jobs + vox_member(grp, blockIdx.x, &local));
+  if (J.health->flag) return;
   const long v = local * kPointThreads + threadIdx.x;
-  if (v < J.n) epilogue_voxel(v, J.n, J.R, J.crss, J.acc, J.sigma, *as_const_mem(J.C));
+  if (v < J.n) epilogue_voxel(v, J.n, J.R, J.crss, J.acc, J.sigma, *as_const_mem(J.C), J.health);
 }
 
 // ---------------------------------------------------------------------------
@@ -795,20 +926,23 @@ __global__ void k_taylor_batch(long n, const double* __restrict__ R, const doubl
 
 // ---------------------------------------------------------------------------
 hipError_t launch_prologue(int prec, long n, const double* R, const double* crss, const StepConst* C, void* field,
-                           double* sigma, double* lambda, double* partial, hipStream_t st) {
+                           double* sigma, double* lambda, double* partial, HealthRecord* health, hipStream_t st) {
   const int nb = point_blocks(n);
   if (prec == 0)
     hipLaunchKernelGGL(k_prologue<double>, dim3(nb), dim3(kPointThreads), 0, st, n, R, crss, C, (double*)field,
-                       sigma, lambda, partial);
+                       sigma, lambda, partial, health);
   else
     hipLaunchKernelGGL(k_prologue<float>, dim3(nb), dim3(kPointThreads), 0, st, n, R, crss, C, (float*)field,
-                       sigma, lambda, partial);
+                       sigma, lambda, partial, health);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_secant, dim3(nb), dim3(kPointThreads), 0, st, n, R, crss, sigma, C, partial, health);
   return hipGetLastError();
 }
 
 hipError_t launch_L0_finalize(long n, int nblocks, const double* partial, StepConst* C, GreenConst<double>* g64,
-                              GreenConst<float>* g32, PrologueRecord* rec, hipStream_t st) {
-  hipLaunchKernelGGL(k_L0_finalize, dim3(1), dim3(256), 0, st, n, nblocks, partial, C, g64, g32, rec);
+                              GreenConst<float>* g32, PrologueRecord* rec, HealthRecord* health, hipStream_t st) {
+  hipLaunchKernelGGL(k_L0_finalize, dim3(1), dim3(256), 0, st, n, nblocks, partial, C, g64, g32, rec, health);
   return hipGetLastError();
 }
 
@@ -819,11 +953,11 @@ hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenCon
 
 hipError_t launch_nr(int prec, int nexp_uniform, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
                      double* lambda, const double* R, const double* crss, const StepConst* C, double* partial,
-                     const int* done, hipStream_t st) {
+                     const LoopState* ls, HealthRecord* health, hipStream_t st) {
   const int nb = nr_blocks(n);
 #define NR_LAUNCH(TT, NN, UU)                                                                                    \
   hipLaunchKernelGGL((k_nr<TT, NN, UU, false>), dim3(nb), dim3(kNrThreads), 0, st, n, (const TT*)dtil, (TT*)phi, \
-                     sigma, lambda, R, crss, C, partial, done, VoxGroup{})
+                     sigma, lambda, R, crss, C, partial, ls, health, VoxGroup{})
 #define NR_N(TT, UU)                                 \
   switch (nexp_uniform) {                             \
     case 5: NR_LAUNCH(TT, 5, UU); break;              \
@@ -843,7 +977,7 @@ hipError_t launch_nr(int prec, int nexp_uniform, bool uniform_tau, long n, const
   // fold the per-block partials in a fixed order: nb -> ceil(nb / 256) rows
   const int nb2 = (nb + 255) / 256;
   hipLaunchKernelGGL(k_fold_partials, dim3(nb2), dim3(256), 0, st, partial, nb, partial + (long)nb * kPartialStride,
-                     done);
+                     ls ? &ls->done : nullptr);
   return hipGetLastError();
 }
 
@@ -858,8 +992,9 @@ hipError_t launch_iter_finalize(long n, const double* partial, const StepConst* 
 }
 
 hipError_t launch_epilogue(long n, double* R, double* crss, double* acc, const double* sigma, const StepConst* C,
-                           hipStream_t st) {
-  hipLaunchKernelGGL(k_epilogue, dim3(point_blocks(n)), dim3(kPointThreads), 0, st, n, R, crss, acc, sigma, C);
+                           HealthRecord* health, hipStream_t st) {
+  hipLaunchKernelGGL(k_epilogue, dim3(point_blocks(n)), dim3(kPointThreads), 0, st, n, R, crss, acc, sigma, C,
+                     health);
   return hipGetLastError();
 }
 
@@ -913,6 +1048,9 @@ hipError_t launch_prologue_group(int prec, const VoxGroup& grp, long blocks, hip
     hipLaunchKernelGGL(k_prologue_b<double>, dim3((unsigned)blocks), dim3(kPointThreads), 0, st, grp);
   else
     hipLaunchKernelGGL(k_prologue_b<float>, dim3((unsigned)blocks), dim3(kPointThreads), 0, st, grp);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_secant_b, dim3((unsigned)blocks), dim3(kPointThreads), 0, st, grp);
   return hipGetLastError();
 }
 hipError_t launch_L0_finalize_group(const VoxGroup& grp, hipStream_t st) {
@@ -927,7 +1065,8 @@ hipError_t launch_nr_group(int prec, int nexp_uniform, bool uniform_tau, const V
                            hipStream_t st) {
 #define NR_LAUNCH(TT, NN, UU)                                                                                   \
   hipLaunchKernelGGL((k_nr<TT, NN, UU, true>), dim3((unsigned)blocks), dim3(kNrThreads), 0, st, 0L,             \
-                     (const TT*)nullptr, (TT*)nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, grp)
+                     (const TT*)nullptr, (TT*)nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, \
+                     nullptr, grp)
 #define NR_N(TT, UU)                                 \
   switch (nexp_uniform) {                             \
     case 5: NR_LAUNCH(TT, 5, UU); break;              \
