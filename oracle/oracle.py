@@ -57,7 +57,8 @@ def load():
     sig = {
         "vo_eigen_basis": [dp], "vo_schmid_basis": [dp, dp],
         "vo_eigenrep_from_matrix": [dp, dp], "vo_matrix_from_eigenrep": [dp, dp],
-        "vo_update_schmid": [dp, dp], "vo_lu_solve_fullpiv": [dp, dp, dp], "vo_inverse": [C.c_int, dp],
+        "vo_update_schmid": [dp, dp], "vo_update_schmid_all": [C.c_long, dp, dp],
+        "vo_lu_solve_fullpiv": [dp, dp, dp], "vo_inverse": [C.c_int, dp],
         "vo_get_max_rss": [dp, dp, dp], "vo_sachs": [dp, dp, dp, dp],
         "vo_taylor_nr": [dp, dp, dp, dp, ip, dp, C.c_double, C.c_double, C.c_int, dp],
         "vo_constrained_nr": [dp, dp, dp, dp, ip, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int, dp, dp],
@@ -266,8 +267,11 @@ def build_L0(dims, material, R, crss, sigma):
 
 
 def schmid_all(R):
-    R = _f(R).reshape(-1, 9)
-    return np.stack([update_schmid(r) for r in R])
+    """Schmid tensors of every voxel, N x 12 x 5."""
+    R = np.ascontiguousarray(_f(R).reshape(-1, 9))
+    m = np.empty((R.shape[0], 60))
+    load().vo_update_schmid_all(R.shape[0], _p(R), _p(m))
+    return m.reshape(-1, 12, 5)
 
 
 def strain_step(dims, material, R, crss, edot, max_iter, dt, tol, acc=None, lengths=(1.0, 1.0, 1.0),

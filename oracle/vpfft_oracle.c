@@ -163,6 +163,14 @@ void vo_update_schmid(const double R[9], double m[60]) {
   }
 }
 
+/* vo_update_schmid for n voxels (R n x 9, m n x 60), MaterialGrid.cpp:107-124. */
+void vo_update_schmid_all(long n, const double *R, double *m) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (long v = 0; v < n; v++) vo_update_schmid(R + 9 * v, m + 60 * v);
+}
+
 /* x^n for integer n >= 0, correctly rounded: binary powering in double-double
  * (products split exactly with fma), one final rounding. */
 static inline void dd_mul(double ah, double al, double bh, double bl, double *rh, double *rl) {

@@ -50,6 +50,7 @@ void vo_schmid_basis(double sym[VO_NSLIP * 9], double anti[VO_NSLIP * 9]);
 void vo_eigenrep_from_matrix(const double M[9], double v[5]);
 void vo_matrix_from_eigenrep(const double v[5], double M[9]);
 void vo_update_schmid(const double R[9], double m[VO_NSLIP * 5]);
+void vo_update_schmid_all(long n, const double *R, double *m);
 
 /* ---- small dense linear algebra ---- */
 double vo_ipow(double x, int n);
