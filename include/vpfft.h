@@ -149,6 +149,20 @@ int vpfft_run_vpfft(void* grid, const double edot[5], int max_iter, int num_step
  * (overwrites the polarization field). */
 int vpfft_strain_rate_evolution(void* grid, const double L0[25], const double* phi_in, double* dtilde_out);
 
+/* The same transform without a grid and without allocating device memory
+ * (no reference counterpart; for callers that cannot allocate -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * d_phi and d_dtilde are device arrays, 5 x N doubles each, N = nx*ny*nz
+ * (precision VPFFT_MIXED transforms in FP32 like a mixed grid).  The per-call
+ * constants, twiddles and half spectrum live in a static device scratch array
+ * of 32 MiB (each GPU the library runs on holds one until the process ends):
+ * a grid whose parts do not fit (larger than about 64^3) is refused with
+ * VPFFT_E_INVALID.  Runs on `stream` (NULL: the default stream) and returns
+ * when the stream has finished it.  Calls on one GPU share its scratch array:
+ * calls that overlap on one GPU (several host threads or streams) race. */
+int vpfft_strain_rate_evolution_noalloc(int nx, int ny, int nz, const double voxel_length[3], int precision,
+                                        const double L0[25], const double* d_phi, double* d_dtilde, void* stream);
+
 /* Solvers::SolveConstrainedConstitutiveEquations over a batch of n voxels
  * (Solvers.h:90-102, Solvers.cpp:111-222).  Per voxel (AoS): orientation
  * n x 9 (Schmid tensors are R m0 R^T, MaterialGrid.cpp:121), crss n x 12,

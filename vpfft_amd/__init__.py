@@ -126,7 +126,8 @@ EXPORTED = [
     "vpfft_solve_taylor_batch", "vpfft_generate_voronoi", "vpfft_last_error", "vpfft_version",
     "vpfft_grid_set_profiling", "vpfft_grid_kernel_times", "vpfft_kernel_name",
     "vpfft_batch_create", "vpfft_batch_destroy", "vpfft_batch_set_tuning", "vpfft_batch_block_shape",
-    "vpfft_batch_run_strain_step", "vpfft_grid_set_block_shape", "vpfft_grid_bytes", "vpfft_grid_save_state", "vpfft_grid_restore_state",
+    "vpfft_batch_run_strain_step", "vpfft_grid_set_block_shape", "vpfft_grid_bytes", "vpfft_grid_save_state",
+    "vpfft_grid_restore_state", "vpfft_strain_rate_evolution_noalloc",
     # include/vpfft_io.h
     "vpfft_deck_read", "vpfft_deck_parse", "vpfft_deck_material", "vpfft_deck_strain_rate",
     "vpfft_two_grain_init", "vpfft_euler_deg_to_matrix", "vpfft_read_euler_file", "vpfft_write_euler_file",
@@ -159,6 +160,8 @@ def load_library(path: Optional[str] = None):
     lib.vpfft_run_vpfft.argtypes = [vp, dp, C.c_int, C.c_int, C.c_double, C.c_double, dp,
                                     C.POINTER(vpfft_step_info), C.c_char_p]
     lib.vpfft_strain_rate_evolution.argtypes = [vp, dp, dp, dp]
+    lib.vpfft_strain_rate_evolution_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int, dp, C.c_void_p,
+                                                        C.c_void_p, C.c_void_p]
     lib.vpfft_solve_constrained_batch.argtypes = [C.c_long, C.POINTER(vpfft_material), dp, dp, dp, dp, dp, dp, dp,
                                                   dp, dp, C.POINTER(C.c_int)]
     lib.vpfft_solve_taylor_batch.argtypes = [C.c_long, C.POINTER(vpfft_material), dp, dp, dp, dp, dp]

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <utility>
 #include <string>
 #include <tuple>
@@ -1285,6 +1286,72 @@ static int batch_create(Batch** out) {
   return VPFFT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of vpfft_strain_rate_evolution_noalloc: the per-call constants,
+// the twiddles and the half spectrum of one call (and, in the mixed form, the
+// FP32 copies of the input and output fields), carved out of one static
+// device array so the call allocates nothing.  Every GPU the library runs on
+// holds its own copy until the process ends; calls on one GPU share it.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices];  // the array's device address, per device
+
+// The current device's copy of the array (hipGetSymbolAddress on the first
+// call on that device).  Host code uses this address, never the array's name.
+static int scratch_address(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(VPFFT_E_DEVICE, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(VPFFT_E_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return VPFFT_OK;
+}
+
+// Offsets (256-byte aligned) of one call's parts in the scratch array.
+struct NoAllocLayout {
+  size_t C, g64, g32, tw64[3], tw32[3], spec, phi32, dtil32, total;
+};
+static NoAllocLayout noalloc_layout(int nx, int ny, int nz, int prec) {
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  NoAllocLayout L{};
+  size_t o = 0;
+  L.C = o;
+  o = up(o + sizeof(StepConst));
+  L.g64 = o;
+  o = up(o + sizeof(GreenConst<double>));
+  L.g32 = o;
+  o = up(o + sizeof(GreenConst<float>));
+  const int dims[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; a++) {
+    L.tw64[a] = o;
+    o = up(o + sizeof(double2) * dims[a]);
+    L.tw32[a] = o;
+    o = up(o + sizeof(float2) * dims[a]);
+  }
+  const size_t n = (size_t)nx * ny * nz, nc = (size_t)nx * ny * padded_half(nz);
+  L.spec = o;
+  o = up(o + (prec == 0 ? sizeof(double2) : sizeof(float2)) * 5 * nc);
+  if (prec == 1) {
+    L.phi32 = o;
+    o = up(o + sizeof(float) * 5 * n);
+    L.dtil32 = o;
+    o = up(o + sizeof(float) * 5 * n);
+  }
+  L.total = o;
+  return L;
+}
+
 }  // namespace vpfft
 
 using vpfft::Grid;
@@ -1517,6 +1584,100 @@ int vpfft_strain_rate_evolution(void* grid, const double L0[25], const double* p
   e = hipStreamSynchronize(g->st);
   if (e != hipSuccess) return set_error(VPFFT_E_DEVICE, "%s", hipGetErrorString(e));
   return vpfft::download_field5(g, g->dtil, dtilde_out, g->dtil_prec);
+}
+
+int vpfft_strain_rate_evolution_noalloc(int nx, int ny, int nz, const double voxel_length[3], int precision,
+                                        const double L0[25], const double* d_phi, double* d_dtilde, void* stream) {
+  using namespace vpfft;
+  if (!voxel_length || !L0 || !d_phi || !d_dtilde) return set_error(VPFFT_E_INVALID, "null argument");
+  if (nx < 2 || ny < 2 || nz < 2 || nx > 4096 || ny > 4096 || nz > 4096)
+    return set_error(VPFFT_E_INVALID, "grid edges must be in [2, 4096]");
+  if (precision != VPFFT_FP64 && precision != VPFFT_MIXED)
+    return set_error(VPFFT_E_INVALID, "precision must be VPFFT_FP64 or VPFFT_MIXED");
+  for (int a = 0; a < 3; a++)
+    if (!(voxel_length[a] > 0) || !std::isfinite(voxel_length[a]))
+      return set_error(VPFFT_E_INVALID, "voxel lengths must be positive");
+  const int dims[3] = {nx, ny, nz};
+  AxisPlan plan[3];
+  for (int a = 0; a < 3; a++)
+    if (!make_axis_plan(dims[a], &plan[a]))
+      return set_error(VPFFT_E_UNSUPPORTED, "grid edge %d has a prime factor other than 2, 3, 5, 7", dims[a]);
+  if ((long)4 * nz * 16 * 2 > 160 * 1024 || (long)ny * 16 * 2 > 160 * 1024 || (long)nx * 16 * 8 * 2 > 160 * 1024)
+    return set_error(VPFFT_E_UNSUPPORTED, "grid edge too long for the LDS-resident FFT");
+  const NoAllocLayout lay = noalloc_layout(nx, ny, nz, precision);
+  if (lay.total > kNoAllocScratchBytes)
+    return set_error(VPFFT_E_INVALID,
+                     "a %d x %d x %d grid needs %zu bytes of scratch; the no-allocation entry point holds %zu "
+                     "(a 64^3 grid fits in either precision)", nx, ny, nz, lay.total, kNoAllocScratchBytes);
+  void* base = nullptr;
+  int rc = scratch_address(&base);
+  if (rc) return rc;
+  unsigned char* b = static_cast<unsigned char*>(base);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int prec = precision;
+  const long n = (long)nx * ny * nz;
+  // the constants vpfft_strain_rate_evolution derives, from the same inputs
+  StepConst hc;
+  std::memset(&hc, 0, sizeof hc);
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j < 5; j++) hc.L0[j * 5 + i] = 0.5 * (L0[j * 5 + i] + L0[i * 5 + j]);
+  for (int a = 0; a < 3; a++) hc.kscale[a] = 1.0 / ((double)dims[a] * voxel_length[a]);
+  hc.inv_n = 1.0 / (double)n;
+  hc.nx = nx;
+  hc.ny = ny;
+  hc.nz = nz;
+  hc.nzh = nz / 2 + 1;
+  auto* dC = reinterpret_cast<StepConst*>(b + lay.C);
+  auto* g64 = reinterpret_cast<GreenConst<double>*>(b + lay.g64);
+  auto* g32 = reinterpret_cast<GreenConst<float>*>(b + lay.g32);
+  HIP_TRY(hipMemcpyAsync(dC, &hc, sizeof hc, hipMemcpyHostToDevice, st));
+  void* tw[3];
+  std::vector<double2> t64s[3];  // kept until the stream is synchronised below
+  std::vector<float2> t32s[3];
+  for (int a = 0; a < 3; a++) {  // the twiddles vpfft_grid_create makes
+    std::vector<double2>& t64 = t64s[a];
+    std::vector<float2>& t32 = t32s[a];
+    t64.resize(dims[a]);
+    t32.resize(dims[a]);
+    for (int t = 0; t < dims[a]; t++) {
+      const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)t / (long double)dims[a];
+      t64[t].x = (double)cosl(ang);
+      t64[t].y = (double)sinl(ang);
+      t32[t].x = (float)t64[t].x;
+      t32[t].y = (float)t64[t].y;
+    }
+    HIP_TRY(hipMemcpyAsync(b + lay.tw64[a], t64.data(), sizeof(double2) * dims[a], hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b + lay.tw32[a], t32.data(), sizeof(float2) * dims[a], hipMemcpyHostToDevice, st));
+    tw[a] = prec == 0 ? (void*)(b + lay.tw64[a]) : (void*)(b + lay.tw32[a]);
+  }
+  HIP_TRY(launch_derive_from_L0(dC, g64, g32, st));
+  FftGeom geom{};
+  geom.n = n;
+  geom.nc = (long)nx * ny * padded_half(nz);
+  geom.nrows = (long)nx * ny;
+  geom.nx = nx;
+  geom.ny = ny;
+  geom.nz = nz;
+  geom.nzh = nz / 2 + 1;
+  geom.nzp = padded_half(nz);
+  void* spec = b + lay.spec;
+  const void* field = d_phi;
+  void* out = d_dtilde;
+  if (prec == 1) {
+    HIP_TRY(launch_convert(1, b + lay.phi32, 0, d_phi, 5 * n, st));
+    field = b + lay.phi32;
+    out = b + lay.dtil32;
+  }
+  const void* gc = prec == 0 ? (const void*)g64 : (const void*)g32;
+  HIP_TRY(launch_z_r2c(prec, field, spec, geom, plan[2], tw[2], 0, nullptr, st));
+  HIP_TRY(launch_y(prec, spec, geom, plan[1], tw[1], -1, 0, nullptr, st));
+  HIP_TRY(launch_x_green(prec, spec, geom, plan[0], tw[0], gc, 0, nullptr, st));
+  HIP_TRY(launch_y(prec, spec, geom, plan[1], tw[1], +1, 0, nullptr, st));
+  HIP_TRY(launch_z_c2r(prec, spec, out, geom, plan[2], tw[2], 1.0 / (double)n, 0, nullptr, st));
+  if (prec == 1) HIP_TRY(launch_convert(0, d_dtilde, 1, b + lay.dtil32, 5 * n, st));
+  // the host sources of the copies above (hc, the twiddles) live until here
+  HIP_TRY(hipStreamSynchronize(st));
+  return VPFFT_OK;
 }
 
 // ---- per-voxel solver batches --------------------------------------------
