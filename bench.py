@@ -249,7 +249,11 @@ def serving_workload(V, args):
         res, st, _ = S.serve(reqs, hbm_budget_bytes=int(args.serve_budget_gib * (1 << 30)), max_active=64,
                              precision=V.MIXED, fp64_fallback=True, max_iter=args.max_iter, tol=args.batch_tol)
         bad = sum(1 for r in res if r["status"] != 0)
-        out.append({"rate_per_s": rate, "requests": len(reqs), "failed": bad,
+        failures = [{"request": i, "status": V.ERRORS.get(r["status"], r["status"]), "edge": q["edge"],
+                     "grains": q["grains"], "n": q["n"], "increments": len(q["edots"]),
+                     "increments_done": r["increments_done"]}
+                    for i, (q, r) in enumerate(zip(reqs, res)) if r["status"] != 0]
+        out.append({"rate_per_s": rate, "requests": len(reqs), "failed": bad, "failures": failures,
                     "rve_steps_per_s": st["rve_steps_per_s"], "voxel_iters_per_s": st["voxel_iters_per_s"],
                     "p50_latency_s": st["p50_latency_s"], "p99_latency_s": st["p99_latency_s"],
                     "mean_latency_s": st["mean_latency_s"], "wall_s": st["wall_s"],

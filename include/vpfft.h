@@ -181,6 +181,45 @@ int vpfft_solve_constrained_batch(long n, const vpfft_material* mat, const doubl
 int vpfft_solve_taylor_batch(long n, const vpfft_material* mat, const double* orientation, const double* crss,
                              const double edot[5], const double* stress_in, double* stress_out);
 
+/* VPFFT::Solvers (Solvers.h:51-153) over n independent voxels, on the GPU,
+ * with the arguments the reference's per-voxel functions take.  Host
+ * buffers, copied in and out; the call returns when the results are back.
+ * Per voxel (AoS): schmid n x 12 x 5 -- the SchmidtTensors vector, sample
+ * frame; crss n x 12.  One strain rate edot (EigenRep) serves the whole
+ * batch, as one macroscopic strain rate serves every voxel of a grid.  mat
+ * gives GammaDotBase and RateSensitivity (and the Voce parameters and the
+ * hardening matrix for vpfft_solvers_hardening).  Arithmetic follows the
+ * reference operation for operation; x^n with an integer n is the double
+ * rounded once from the exact power (std::pow's glibc result agrees but for
+ * rare halfway cases), and exp is the device library's.  n = 0 does nothing. */
+/* SolveConstitutiveEquations (Solvers.h:51-59, Solvers.cpp:19-101). */
+int vpfft_solvers_taylor(long n, const vpfft_material* mat, const double* schmid, const double* crss,
+                         const double edot[5], const double* stress_in, double eps, double max_rss, int max_it,
+                         double* stress_out);
+/* SolveConstrainedConstitutiveEquations (Solvers.h:67-79, Solvers.cpp:111-222):
+ * lambda, dtilde, stress_in n x 5; L0 the HomogeonousReference, column-major
+ * 25, used as given.  nr_error n: NR_Error_Out, in and out -- left as it came
+ * in when the step budget runs out inside the admissibility back-track
+ * (Solvers.cpp:163-170). */
+int vpfft_solvers_constrained(long n, const vpfft_material* mat, const double* schmid, const double* crss,
+                              const double edot[5], const double* lambda, const double* dtilde, const double L0[25],
+                              const double* stress_in, double eps, double max_rss, int max_it, double* stress_out,
+                              double* nr_error);
+/* SolveSachEquation (Solvers.h:88-90, Solvers.cpp:230-243). */
+int vpfft_solvers_sachs(long n, const double* schmid, const double* crss, const double edot[5], double* out);
+/* ApplyConstitutiveEquations (Solvers.h:106-110, Solvers.cpp:255-274): stress n x 5 -> strain rate n x 5. */
+int vpfft_solvers_apply(long n, const vpfft_material* mat, const double* schmid, const double* crss,
+                        const double* stress, double* strain_rate_out);
+/* CalculateShearRate (Solvers.h:120-125, Solvers.cpp:282-302): anti n x 12 x 9,
+ * the AntiSchmidtTensors (SMatrix3x3, row-major) in the frame of schmid;
+ * shear_rate_out n x 9 (SMatrix3x3). */
+int vpfft_solvers_shear_rate(long n, const vpfft_material* mat, const double* schmid, const double* anti,
+                             const double* crss, const double* stress, double* shear_rate_out);
+/* CalculateHardening (Solvers.h:136-147, Solvers.cpp:314-373): accumulated_shear
+ * n, in and out (AccumulatedShear); crss_out n x 12 (the updated CRSS). */
+int vpfft_solvers_hardening(long n, const vpfft_material* mat, const double* schmid, const double* crss,
+                            const double* stress, double dt, double* accumulated_shear, double* crss_out);
+
 /* Deterministic microstructure (no reference counterpart; the reference's
  * SetRandomMaterialGrid, UnitTester.cpp:174-231, uses rand()): periodic
  * Voronoi tessellation of num_grains seeded uniform centres in [0,N)^3

@@ -1,7 +1,14 @@
 /* vpfft_materialgrid.hpp -- the reference's C++ interface to the strain-step
  * engine, VPFFT::DataStructures::MaterialGrid (Src/MaterialGrid.h:59-577),
- * as a header-only class over the C ABI of vpfft.h.  A reference driver
- * (Src/UnitTester.cpp:384-457) compiles against it with its calls unchanged.
+ * as a header-only class over the C ABI of vpfft.h, with the reference's
+ * value types and slip tables from vpfft_linalg.hpp.  The calls a driver
+ * such as MPI_VPFFT_Test (Src/UnitTester.cpp:384-457) makes on the grid and
+ * on those types -- the constructor, SetVoxelLength, SendDataAndInitialize,
+ * the accessors, RunSingleStrainStep / RunVPFFT, SMatrix3x3::SetZero,
+ * EigenRep(const SMatrix3x3&), FCC_SchmidtBasis::Get -- compile against it
+ * unchanged.  What that driver adds around them (MPI_Init and MPI_Finalize,
+ * RecvDataAndInitialize on other ranks, the reference's own
+ * SetRandomMaterialGrid) has no counterpart here.
  *
  * Differences from the reference, all on purpose:
  *  - fields live in GPU memory.  The accessors read a host mirror that is
@@ -20,53 +27,14 @@
 #pragma once
 
 #include <cstddef>
-#include <stdexcept>
-#include <string>
 #include <utility>
 #include <vector>
 
 #include "vpfft.h"
+#include "vpfft_error.hpp"
+#include "vpfft_linalg.hpp"
 
 namespace VPFFT {
-
-typedef double Float;
-
-/* A status code of the C ABI (VPFFT_E_*) with vpfft_last_error()'s text. */
-class Error : public std::runtime_error {
- public:
-  Error(int code, const std::string& what) : std::runtime_error(what), code_(code) {}
-  int code() const { return code_; }
-
- private:
-  int code_;
-};
-
-inline void check(int rc) {
-  if (rc != VPFFT_OK) throw Error(rc, vpfft_last_error());
-}
-
-namespace LinearAlgebra {
-
-/* The 5-component deviatoric representation, LinearAlgebra.h:298-521 (m[5]). */
-struct EigenRep {
-  Float m[5];
-  EigenRep() : m{0, 0, 0, 0, 0} {}
-  EigenRep(Float a, Float b, Float c, Float d, Float e) : m{a, b, c, d, e} {}
-  Float& operator[](int i) { return m[i]; }
-  const Float& operator[](int i) const { return m[i]; }
-};
-
-/* Row-major 3x3, LinearAlgebra.h:44-115 (m[3][3]). */
-struct SMatrix3x3 {
-  Float m[3][3];
-};
-
-/* Column-major 5x5, LinearAlgebra.h:574-575 (m[j * 5 + i] is row i, column j). */
-struct SMatrix5x5 {
-  Float m[25];
-};
-
-}  // namespace LinearAlgebra
 
 namespace DataStructures {
 

@@ -176,6 +176,12 @@ def max_rss(s, m, tau):
     return load().vo_get_max_rss(_p(_f(s, (5,))), _p(_f(m, (60,))), _p(_f(tau, (12,))))
 
 
+def sachs(m, tau, edot):
+    out = np.empty(5)
+    load().vo_sachs(_p(_f(m, (60,))), _p(_f(tau, (12,))), _p(_f(edot, (5,))), _p(out))
+    return out
+
+
 def taylor_nr(init, m, tau, material, edot, eps=5e-4, max_rss_=2.0, max_it=1000):
     g, n = material_arrays(material)
     out = np.empty(5)

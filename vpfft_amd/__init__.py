@@ -128,6 +128,8 @@ EXPORTED = [
     "vpfft_batch_create", "vpfft_batch_destroy", "vpfft_batch_set_tuning", "vpfft_batch_block_shape",
     "vpfft_batch_run_strain_step", "vpfft_grid_set_block_shape", "vpfft_grid_bytes", "vpfft_grid_save_state",
     "vpfft_grid_restore_state", "vpfft_strain_rate_evolution_noalloc",
+    "vpfft_solvers_taylor", "vpfft_solvers_constrained", "vpfft_solvers_sachs", "vpfft_solvers_apply",
+    "vpfft_solvers_shear_rate", "vpfft_solvers_hardening",
     # include/vpfft_io.h
     "vpfft_deck_read", "vpfft_deck_parse", "vpfft_deck_material", "vpfft_deck_strain_rate",
     "vpfft_two_grain_init", "vpfft_euler_deg_to_matrix", "vpfft_read_euler_file", "vpfft_write_euler_file",
@@ -165,6 +167,14 @@ def load_library(path: Optional[str] = None):
     lib.vpfft_solve_constrained_batch.argtypes = [C.c_long, C.POINTER(vpfft_material), dp, dp, dp, dp, dp, dp, dp,
                                                   dp, dp, C.POINTER(C.c_int)]
     lib.vpfft_solve_taylor_batch.argtypes = [C.c_long, C.POINTER(vpfft_material), dp, dp, dp, dp, dp]
+    mp = C.POINTER(vpfft_material)
+    lib.vpfft_solvers_taylor.argtypes = [C.c_long, mp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int, dp]
+    lib.vpfft_solvers_constrained.argtypes = [C.c_long, mp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double,
+                                              C.c_int, dp, dp]
+    lib.vpfft_solvers_sachs.argtypes = [C.c_long, dp, dp, dp, dp]
+    lib.vpfft_solvers_apply.argtypes = [C.c_long, mp, dp, dp, dp, dp]
+    lib.vpfft_solvers_shear_rate.argtypes = [C.c_long, mp, dp, dp, dp, dp, dp]
+    lib.vpfft_solvers_hardening.argtypes = [C.c_long, mp, dp, dp, dp, C.c_double, dp, dp]
     lib.vpfft_generate_voronoi.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, dp,
                                            C.POINTER(C.c_int32), dp]
     lib.vpfft_grid_set_profiling.argtypes = [vp, C.c_int]
@@ -518,6 +528,116 @@ def solve_taylor_batch(material: Material, R, crss, edot, stress):
     _check(lib.vpfft_solve_taylor_batch(n, C.byref(m), _dptr(R), _dptr(crss), _dptr(edot), _dptr(stress),
                                         _dptr(out)))
     return out
+
+
+class Solvers:
+    """VPFFT::Solvers (Src/Solvers.h:51-153) over batches of voxels on the GPU,
+    with the reference's function names and argument order.  Every per-voxel
+    argument takes a leading batch axis: SchmidtTensors (n, 12, 5), CRSS
+    (n, 12), stresses (n, 5), AntiSchmidtTensors (n, 12, 3, 3).  The strain
+    rate and the material vectors (GammaDotBase, RateSensitivity, ...) are
+    one per call.  A single voxel is a batch of one."""
+
+    @staticmethod
+    def _mat(gdot0, nexp, hmat=None, tau0=None, tau1=None, theta0=None, theta1=None) -> vpfft_material:
+        m = Material(gamma_dot_base=list(np.asarray(gdot0, dtype=np.float64).reshape(NSLIP)),
+                     rate_sensitivity=[int(x) for x in np.asarray(nexp).reshape(NSLIP)])
+        if hmat is not None:
+            m.hardening = np.asarray(hmat, dtype=np.float64).reshape(NSLIP, NSLIP)
+            m.tau0, m.tau1, m.theta0, m.theta1 = (list(np.asarray(v, dtype=np.float64).reshape(NSLIP))
+                                                  for v in (tau0, tau1, theta0, theta1))
+        return m.to_c()
+
+    @staticmethod
+    def _batch(SchmidtTensors, CRSS):
+        m = _f64(SchmidtTensors)
+        n = m.reshape(-1, 60).shape[0]
+        return n, _f64(m, (n, 60)), _f64(CRSS, (n, 12))
+
+    @staticmethod
+    def SolveConstitutiveEquations(InitialStressState, SchmidtTensors, CRSS, GammaDotBase, RateSensitivity,
+                                   StrainRate, EpsilonConvergence, MaxResolvedStress, MaxNumIterations):
+        """Taylor Newton-Raphson (Solvers.cpp:19-101): stress per voxel (n, 5)."""
+        n, m, t = Solvers._batch(SchmidtTensors, CRSS)
+        s0, e = _f64(InitialStressState, (n, 5)), _f64(StrainRate, (5,))
+        out = np.empty((n, 5))
+        mat = Solvers._mat(GammaDotBase, RateSensitivity)
+        _check(load_library().vpfft_solvers_taylor(n, C.byref(mat), _dptr(m), _dptr(t), _dptr(e), _dptr(s0),
+                                                   float(EpsilonConvergence), float(MaxResolvedStress),
+                                                   int(MaxNumIterations), _dptr(out)))
+        return out
+
+    @staticmethod
+    def SolveConstrainedConstitutiveEquations(InitialStressState, SchmidtTensors, CRSS, GammaDotBase,
+                                              RateSensitivity, MacroscopicStrainRate, LagrangeMultiplier,
+                                              LocalDisplacementVariation, HomogeonousReference, EpsilonConvergence,
+                                              MaxResolvedStress, MaxNumIterations, NR_Error_Out):
+        """Constrained Newton-Raphson (Solvers.cpp:111-222).  NR_Error_Out is an
+        (n,) float64 array updated in place, as the reference writes through
+        its pointer (and leaves it alone when the budget runs out in the
+        admissibility back-track).  Returns the stresses (n, 5)."""
+        n, m, t = Solvers._batch(SchmidtTensors, CRSS)
+        s0, e = _f64(InitialStressState, (n, 5)), _f64(MacroscopicStrainRate, (5,))
+        lam, dtl = _f64(LagrangeMultiplier, (n, 5)), _f64(LocalDisplacementVariation, (n, 5))
+        L0 = _f64(HomogeonousReference, (25,))
+        err = NR_Error_Out
+        if not (isinstance(err, np.ndarray) and err.dtype == np.float64 and err.shape == (n,)
+                and err.flags.c_contiguous):
+            raise ValueError("NR_Error_Out must be a contiguous float64 array of shape (n,)")
+        out = np.empty((n, 5))
+        mat = Solvers._mat(GammaDotBase, RateSensitivity)
+        _check(load_library().vpfft_solvers_constrained(n, C.byref(mat), _dptr(m), _dptr(t), _dptr(e), _dptr(lam),
+                                                        _dptr(dtl), _dptr(L0), _dptr(s0), float(EpsilonConvergence),
+                                                        float(MaxResolvedStress), int(MaxNumIterations),
+                                                        _dptr(out), _dptr(err)))
+        return out
+
+    @staticmethod
+    def SolveSachEquation(SchmidtTensors, CRSS, StrainRate):
+        """Sachs guess (Solvers.cpp:230-243): (n, 5)."""
+        n, m, t = Solvers._batch(SchmidtTensors, CRSS)
+        e = _f64(StrainRate, (5,))
+        out = np.empty((n, 5))
+        _check(load_library().vpfft_solvers_sachs(n, _dptr(m), _dptr(t), _dptr(e), _dptr(out)))
+        return out
+
+    @staticmethod
+    def ApplyConstitutiveEquations(StressState, SchmidtTensors, CRSS, GammaDotBase, RateSensitivity):
+        """Strain rate of given stresses (Solvers.cpp:255-274): (n, 5)."""
+        n, m, t = Solvers._batch(SchmidtTensors, CRSS)
+        s = _f64(StressState, (n, 5))
+        out = np.empty((n, 5))
+        mat = Solvers._mat(GammaDotBase, RateSensitivity)
+        _check(load_library().vpfft_solvers_apply(n, C.byref(mat), _dptr(m), _dptr(t), _dptr(s), _dptr(out)))
+        return out
+
+    @staticmethod
+    def CalculateShearRate(StressState, SchmidtTensors, AntiSchmidtTensors, CRSS, GammaDotBase, RateSensitivity):
+        """Shear rate (Solvers.cpp:282-302): (n, 3, 3)."""
+        n, m, t = Solvers._batch(SchmidtTensors, CRSS)
+        s, a = _f64(StressState, (n, 5)), _f64(AntiSchmidtTensors, (n, 108))
+        out = np.empty((n, 3, 3))
+        mat = Solvers._mat(GammaDotBase, RateSensitivity)
+        _check(load_library().vpfft_solvers_shear_rate(n, C.byref(mat), _dptr(m), _dptr(a), _dptr(t), _dptr(s),
+                                                       _dptr(out)))
+        return out
+
+    @staticmethod
+    def CalculateHardening(AccumulatedShear, TimeStep, StressState, SchmidtTensors, CRSS, GammaDotBase,
+                           RateSensitivity, HardeningMatrix, Tau0, Tau1, Theta0, Theta1):
+        """Voce hardening as written (Solvers.cpp:314-373).  AccumulatedShear
+        is an (n,) float64 array updated in place; returns the CRSS (n, 12)."""
+        n, m, t = Solvers._batch(SchmidtTensors, CRSS)
+        acc = AccumulatedShear
+        if not (isinstance(acc, np.ndarray) and acc.dtype == np.float64 and acc.shape == (n,)
+                and acc.flags.c_contiguous):
+            raise ValueError("AccumulatedShear must be a contiguous float64 array of shape (n,)")
+        s = _f64(StressState, (n, 5))
+        out = np.empty((n, 12))
+        mat = Solvers._mat(GammaDotBase, RateSensitivity, HardeningMatrix, Tau0, Tau1, Theta0, Theta1)
+        _check(load_library().vpfft_solvers_hardening(n, C.byref(mat), _dptr(m), _dptr(t), _dptr(s),
+                                                      float(TimeStep), _dptr(acc), _dptr(out)))
+        return out
 
 
 # ---------------------------------------------------------------------------

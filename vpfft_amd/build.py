@@ -62,10 +62,12 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(RUNNER + ".tmp", RUNNER)
-    # the C++ MaterialGrid facade's test driver (tests/cpp), linked the same way
-    check_src = os.path.join(os.path.dirname(HERE), "tests", "cpp", "facade_check.cpp")
-    if os.path.exists(check_src):
-        out = os.path.join(os.path.dirname(check_src), "facade_check")
+    # the C++ facades' test drivers (tests/cpp), linked the same way
+    for name in ("facade_check", "solvers_check"):
+        check_src = os.path.join(os.path.dirname(HERE), "tests", "cpp", name + ".cpp")
+        if not os.path.exists(check_src):
+            continue
+        out = os.path.join(os.path.dirname(check_src), name)
         cmd = [hipcc(), "-O2", "-std=c++17", "-Wall", "-x", "c++", check_src, "-o", out + ".tmp",
                "-I", os.path.join(os.path.dirname(HERE), "include"), "-L", HERE, "-lvpfft",
                "-Wl,-rpath,$ORIGIN/../../vpfft_amd"]

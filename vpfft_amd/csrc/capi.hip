@@ -1794,6 +1794,140 @@ int vpfft_solve_taylor_batch(long n, const vpfft_material* mat, const double* or
   return VPFFT_OK;
 }
 
+namespace {
+// Constants of one Solvers call: the material, the strain rate, the
+// admissibility bound and L0 as the caller gives them (not symmetrised).
+void solver_const(const vpfft_material* mat, const double* L0, const double* edot, double max_rss,
+                  vpfft::StepConst* hc) {
+  std::memset(hc, 0, sizeof *hc);
+  if (mat) {
+    hc->uniform_n = 1;
+    for (int k = 0; k < vpfft::kSlips; k++) {
+      hc->gdot0[k] = mat->gamma_dot_base[k];
+      hc->nexp[k] = mat->rate_sensitivity[k];
+      if (hc->nexp[k] != hc->nexp[0]) hc->uniform_n = 0;
+      hc->tau0[k] = mat->tau0[k];
+      hc->tau1[k] = mat->tau1[k];
+      hc->theta0[k] = mat->theta0[k];
+      hc->theta1[k] = mat->theta1[k];
+    }
+    for (int e = 0; e < vpfft::kSlips * vpfft::kSlips; e++) hc->hmat[e] = mat->hardening[e];
+  }
+  if (edot)
+    for (int i = 0; i < 5; i++) hc->edot[i] = edot[i];
+  if (L0)
+    for (int e = 0; e < 25; e++) hc->L0[e] = L0[e];
+  hc->imposed_max_rss = max_rss;
+}
+
+// One host array of a Solvers call: n x width doubles, copied in (in),
+// copied back (out), or both; width 0 marks an array the operation does not
+// take.
+struct SolverArg {
+  const double* in;
+  double* out;
+  int width;
+  double* dev;
+};
+
+int solvers_run(const char* what, int op, long n, const vpfft_material* mat, const double* L0, const double* edot,
+                double eps, double max_rss, int max_it, double dt, SolverArg* args, int nargs) {
+  if (n < 0) return set_error(VPFFT_E_INVALID, "%s: n = %ld", what, n);
+  if (n == 0) return VPFFT_OK;
+  for (int a = 0; a < nargs; a++)
+    if (args[a].width && !args[a].in && !args[a].out)
+      return set_error(VPFFT_E_INVALID, "%s: null array argument", what);
+  if (mat) {
+    int rc = vpfft::check_material(mat);
+    if (rc) return rc;
+  }
+  vpfft::StepConst hc;
+  solver_const(mat, L0, edot, max_rss, &hc);
+  BatchBufs B;
+  auto* dC = (vpfft::StepConst*)B.get(sizeof hc);
+  if (!dC) return set_error(VPFFT_E_NOMEM, "%s: allocation failed", what);
+  hipError_t e = hipMemcpy(dC, &hc, sizeof hc, hipMemcpyHostToDevice);
+  for (int a = 0; a < nargs && e == hipSuccess; a++) {
+    if (!args[a].width) continue;
+    const size_t bytes = sizeof(double) * args[a].width * (size_t)n;
+    args[a].dev = (double*)B.get(bytes);
+    if (!args[a].dev) return set_error(VPFFT_E_NOMEM, "%s: allocation of %zu bytes failed", what, bytes);
+    if (args[a].in) e = hipMemcpy(args[a].dev, args[a].in, bytes, hipMemcpyHostToDevice);
+  }
+  // slots, in the order of launch_solvers: schmid, anti, crss, a5, b5, c5, acc, err, out
+  double* d[9] = {nullptr};
+  for (int a = 0; a < nargs && a < 9; a++) d[a] = args[a].dev;
+  if (e == hipSuccess)
+    e = vpfft::launch_solvers(op, n, d[0], d[1], d[2], dC, d[3], d[4], d[5], eps, max_rss, max_it, dt, d[6], d[7],
+                              d[8], nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  for (int a = 0; a < nargs && e == hipSuccess; a++)
+    if (args[a].width && args[a].out) e = hipMemcpy(args[a].out, args[a].dev, sizeof(double) * args[a].width * (size_t)n,
+                                   hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return set_error(VPFFT_E_DEVICE, "%s: %s", what, hipGetErrorString(e));
+  return VPFFT_OK;
+}
+
+constexpr SolverArg kUnused = {nullptr, nullptr, 0, nullptr};
+SolverArg in_arg(const double* p, int w) { return {p, nullptr, w, nullptr}; }
+SolverArg out_arg(double* p, int w) { return {nullptr, p, w, nullptr}; }
+SolverArg inout_arg(double* p, int w) { return {p, p, w, nullptr}; }
+}  // namespace
+
+int vpfft_solvers_taylor(long n, const vpfft_material* mat, const double* schmid, const double* crss,
+                         const double edot[5], const double* stress_in, double eps, double max_rss, int max_it,
+                         double* stress_out) {
+  if (!mat || !edot) return set_error(VPFFT_E_INVALID, "vpfft_solvers_taylor: null argument");
+  SolverArg a[9] = {in_arg(schmid, 60), kUnused, in_arg(crss, 12), in_arg(stress_in, 5), kUnused,
+                    kUnused, kUnused, kUnused, out_arg(stress_out, 5)};
+  return solvers_run("vpfft_solvers_taylor", vpfft::SOLVER_TAYLOR, n, mat, nullptr, edot, eps, max_rss, max_it, 0.0,
+                     a, 9);
+}
+
+int vpfft_solvers_constrained(long n, const vpfft_material* mat, const double* schmid, const double* crss,
+                              const double edot[5], const double* lambda, const double* dtilde, const double L0[25],
+                              const double* stress_in, double eps, double max_rss, int max_it, double* stress_out,
+                              double* nr_error) {
+  if (!mat || !edot || !L0) return set_error(VPFFT_E_INVALID, "vpfft_solvers_constrained: null argument");
+  SolverArg a[9] = {in_arg(schmid, 60), kUnused, in_arg(crss, 12), in_arg(stress_in, 5), in_arg(lambda, 5),
+                    in_arg(dtilde, 5), kUnused, inout_arg(nr_error, 1), out_arg(stress_out, 5)};
+  return solvers_run("vpfft_solvers_constrained", vpfft::SOLVER_CONSTRAINED, n, mat, L0, edot, eps, max_rss, max_it,
+                     0.0, a, 9);
+}
+
+int vpfft_solvers_sachs(long n, const double* schmid, const double* crss, const double edot[5], double* out) {
+  if (!edot) return set_error(VPFFT_E_INVALID, "vpfft_solvers_sachs: null argument");
+  SolverArg a[9] = {in_arg(schmid, 60), kUnused, in_arg(crss, 12), kUnused, kUnused,
+                    kUnused, kUnused, kUnused, out_arg(out, 5)};
+  return solvers_run("vpfft_solvers_sachs", vpfft::SOLVER_SACHS, n, nullptr, nullptr, edot, 0.0, 0.0, 0, 0.0, a, 9);
+}
+
+int vpfft_solvers_apply(long n, const vpfft_material* mat, const double* schmid, const double* crss,
+                        const double* stress, double* strain_rate_out) {
+  if (!mat) return set_error(VPFFT_E_INVALID, "vpfft_solvers_apply: null argument");
+  SolverArg a[9] = {in_arg(schmid, 60), kUnused, in_arg(crss, 12), in_arg(stress, 5), kUnused,
+                    kUnused, kUnused, kUnused, out_arg(strain_rate_out, 5)};
+  return solvers_run("vpfft_solvers_apply", vpfft::SOLVER_APPLY, n, mat, nullptr, nullptr, 0.0, 0.0, 0, 0.0, a, 9);
+}
+
+int vpfft_solvers_shear_rate(long n, const vpfft_material* mat, const double* schmid, const double* anti,
+                             const double* crss, const double* stress, double* shear_rate_out) {
+  if (!mat) return set_error(VPFFT_E_INVALID, "vpfft_solvers_shear_rate: null argument");
+  SolverArg a[9] = {in_arg(schmid, 60), in_arg(anti, 108), in_arg(crss, 12), in_arg(stress, 5), kUnused,
+                    kUnused, kUnused, kUnused, out_arg(shear_rate_out, 9)};
+  return solvers_run("vpfft_solvers_shear_rate", vpfft::SOLVER_SHEAR_RATE, n, mat, nullptr, nullptr, 0.0, 0.0, 0,
+                     0.0, a, 9);
+}
+
+int vpfft_solvers_hardening(long n, const vpfft_material* mat, const double* schmid, const double* crss,
+                            const double* stress, double dt, double* accumulated_shear, double* crss_out) {
+  if (!mat) return set_error(VPFFT_E_INVALID, "vpfft_solvers_hardening: null argument");
+  SolverArg a[9] = {in_arg(schmid, 60), kUnused, in_arg(crss, 12), in_arg(stress, 5), kUnused,
+                    kUnused, inout_arg(accumulated_shear, 1), kUnused, out_arg(crss_out, 12)};
+  return solvers_run("vpfft_solvers_hardening", vpfft::SOLVER_HARDENING, n, mat, nullptr, nullptr, 0.0, 0.0, 0, dt,
+                     a, 9);
+}
+
 int vpfft_batch_create(void** out_batch) {
   if (!out_batch) return set_error(VPFFT_E_INVALID, "null argument");
   vpfft::Batch* b = nullptr;

@@ -1038,6 +1038,105 @@ __device__ __forceinline__ void taylor_nr(const double init[5], const double m[k
   if (steps_out) *steps_out = steps;
 }
 
+// Solvers::SolveConstrainedConstitutiveEquations (Solvers.cpp:111-222) in the
+// reference's own form, for Schmid tensors given in the sample frame: the
+// Newton matrix J' = -I - L0 J (:202-204) solved by the complete-pivoting LU,
+// operation for operation as the oracle restates it.  C supplies gdot0, the
+// exponents, edot and L0 (column-major).  *err_out is left untouched when the
+// budget runs out in the admissibility back-track (:166-170).
+__device__ __forceinline__ void constrained_literal(const double init[5], const double m[kSlips][5],
+                                                    const double tau[kSlips], const StepConst& C, const double lam[5],
+                                                    const double dtl[5], double eps, double max_rss, int max_it,
+                                                    double out[5], double* err_out) {
+#pragma clang fp contract(off)
+  double cur[5], saved[5], nw[5] = {-1.0, -3.0, -7.0, -9.0, -11.0};
+  double rel = 1e5;
+#pragma unroll
+  for (int i = 0; i < 5; i++) { cur[i] = init[i]; saved[i] = init[i]; }
+  int rem = max_it;
+  while (rem > 0) {
+    double rss[kSlips];
+    bool ok;
+    do {
+      ok = true;
+#pragma unroll
+      for (int k = 0; k < kSlips; k++) {
+        rss[k] = dot5(m[k], cur) / tau[k];
+        if (fabs(rss[k]) > max_rss) ok = false;
+      }
+      if (!ok) {
+#pragma unroll
+        for (int i = 0; i < 5; i++) cur[i] = saved[i] + (cur[i] - saved[i]) * 0.2;
+      }
+      rem--;
+      if (rem < 0) {
+#pragma unroll
+        for (int i = 0; i < 5; i++) out[i] = cur[i];
+        return;
+      }
+    } while (!ok);
+    double S[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, J[25];
+#pragma unroll
+    for (int e = 0; e < 25; e++) J[e] = 0.0;
+#pragma unroll
+    for (int k = 0; k < kSlips; k++) {
+      const int n = C.uniform_n ? C.nexp[0] : C.nexp[k];
+      const double g = n != 1 ? C.gdot0[k] * ipow_cr(fabs(rss[k]), n - 1) : C.gdot0[k];
+#pragma unroll
+      for (int i = 0; i < 5; i++) S[i] += m[k][i] * g * rss[k];
+#pragma unroll
+      for (int j = 0; j < 5; j++)
+#pragma unroll
+        for (int i = 0; i < 5; i++) J[j * 5 + i] += m[k][i] * m[k][j] * (double)n * g / tau[k];
+    }
+    double Jp[25];
+#pragma unroll
+    for (int j = 0; j < 5; j++)
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        double r = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; k++) r += C.L0[k * 5 + i] * J[j * 5 + k];
+        Jp[j * 5 + i] = (i == j ? -1.0 : -0.0) - r;
+      }
+    double t[5], res[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) t[i] = S[i] - C.edot[i] - dtl[i];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      double r = 0.0;
+#pragma unroll
+      for (int j = 0; j < 5; j++) r += C.L0[j * 5 + i] * t[j];
+      res[i] = cur[i] - lam[i] + r;
+    }
+#pragma unroll
+    for (int i = 0; i < 5; i++) saved[i] = cur[i];
+    double delta[5], diff[5], avg[5];
+    lu_fullpiv(Jp, res, delta);
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      nw[i] = cur[i] + delta[i];
+      diff[i] = nw[i] - saved[i];
+      avg[i] = (nw[i] + saved[i]) * 0.5;
+    }
+    rel = norm5(diff) / norm5(avg);
+#pragma unroll
+    for (int i = 0; i < 5; i++) cur[i] = nw[i];
+    if (rel < eps) break;
+  }
+  *err_out = rel;
+#pragma unroll
+  for (int i = 0; i < 5; i++) out[i] = nw[i];
+}
+
+// gdot0 |r|^n sgn(r) of one slip system for the rate laws of Solvers.cpp:255-373
+// (n > 0; gdot0 alone otherwise), r = m . s / tau, with the correctly rounded power.
+__device__ __forceinline__ double slip_rate(const StepConst& C, int k, double rss) {
+#pragma clang fp contract(off)
+  const int n = C.uniform_n ? C.nexp[0] : C.nexp[k];
+  return n > 0 ? C.gdot0[k] * ipow_cr(fabs(rss), n) : C.gdot0[k];
+}
+
 }  // namespace faithful
 
 }  // namespace vpfft

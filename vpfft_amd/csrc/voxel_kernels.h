@@ -117,6 +117,18 @@ hipError_t launch_constrained_batch(long n, const double* R, const double* crss,
 hipError_t launch_taylor_batch(long n, const double* R, const double* crss, const StepConst* C, const double* s_in,
                                double* s_out, hipStream_t st);
 
+// VPFFT::Solvers over n voxels given by their sample-frame Schmid tensors
+// (n x 60) and CRSS (n x 12); C: material, strain rate, L0.  a5/b5/c5 are
+// n x 5 inputs whose meaning depends on op (see k_solvers).
+enum SolverOp {
+  SOLVER_TAYLOR = 0, SOLVER_CONSTRAINED = 1, SOLVER_SACHS = 2, SOLVER_APPLY = 3, SOLVER_SHEAR_RATE = 4,
+  SOLVER_HARDENING = 5
+};
+hipError_t launch_solvers(int op, long n, const double* schmid, const double* anti, const double* crss,
+                          const StepConst* C, const double* a5, const double* b5, const double* c5, double eps,
+                          double max_rss, int max_it, double dt, double* acc, double* err, double* out,
+                          hipStream_t st);
+
 // One grid of a batched pointwise launch.  dtil / phi are in the group's
 // transform precision; partial holds nr_blocks(n) rows of kPartialStride
 // (and of 18 for the prologue).

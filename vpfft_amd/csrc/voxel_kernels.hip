@@ -100,18 +100,41 @@ __device__ __forceinline__ void block_sum(double (&v)[W], double (&vmax)[M], dou
   }
 }
 
-// Reduce W sums and M maxima over one wave in a fixed order; lane 0 writes
-// them to out[0..W+M).
+// A double moved across lanes by one DPP pattern (two 32-bit moves); lanes
+// of rows outside ROWMASK read 0.
+template <int CTRL, int ROWMASK = 0xf>
+__device__ __forceinline__ double dpp64(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum (or maximum) over one full wave in a fixed order, by DPP moves: within
+// quads, within rows of 16 (rotations by 4 and 8), then rows 1 and 3 take
+// lane 15 / 47 and rows 2 and 3 lane 31.  Lane 63 holds the result.
+template <bool MAX>
+__device__ __forceinline__ double wave_reduce(double v) {
+  auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+  v = op(v, dpp64<0xB1>(v));        // quad_perm [1,0,3,2]
+  v = op(v, dpp64<0x4E>(v));        // quad_perm [2,3,0,1]
+  v = op(v, dpp64<0x124>(v));       // row_ror:4
+  v = op(v, dpp64<0x128>(v));       // row_ror:8
+  v = op(v, dpp64<0x142, 0xa>(v));  // row_bcast:15
+  v = op(v, dpp64<0x143, 0xc>(v));  // row_bcast:31
+  return v;
+}
+
+// Reduce W sums and M maxima over one wave in a fixed order; lane 63 writes
+// them to out[0..W+M).  The maxima must be of values >= 0: lanes of rows a
+// broadcast leaves out read 0.
 template <int W, int M>
 __device__ __forceinline__ void wave_sum(double (&v)[W], double (&vmax)[M], double* out) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
+  for (int w = 0; w < W; w++) v[w] = wave_reduce<false>(v[w]);
 #pragma unroll
-    for (int w = 0; w < W; w++) v[w] += __shfl_xor(v[w], off);
-#pragma unroll
-    for (int w = 0; w < M; w++) vmax[w] = fmax(vmax[w], __shfl_xor(vmax[w], off));
-  }
-  if ((threadIdx.x & 63) == 0) {
+  for (int w = 0; w < M; w++) vmax[w] = wave_reduce<true>(vmax[w]);
+  if ((threadIdx.x & 63) == 63) {
 #pragma unroll
     for (int w = 0; w < W; w++) out[w] = v[w];
 #pragma unroll
@@ -621,12 +644,10 @@ k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restr
     }
   }
   // lane efficiency of the Newton loop: a wave runs as many steps as its
-  // slowest lane
-  double wmax = mx[1];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
-  if ((threadIdx.x & 63) == 0) acc[14] = 64.0 * wmax;
+  // slowest lane, the wave's maximum of the steps (slot 14 is 0 in every
+  // lane before the reduction and 64 x that maximum after it)
   wave_sum<15>(acc, mx, partial + blk * kPartialStride);
+  if ((threadIdx.x & 63) == 63) partial[blk * kPartialStride + 14] = 64.0 * mx[1];
 }
 
 
@@ -922,6 +943,147 @@ __global__ void k_taylor_batch(long n, const double* __restrict__ R, const doubl
     faithful::taylor_nr(s, m, tau, C, 5e-4, 1000, s);
     for (int i = 0; i < 5; i++) s_out[5 * v + i] = s[i];
   }
+}
+
+// ---------------------------------------------------------------------------
+// VPFFT::Solvers (Src/Solvers.h:51-153) over batches of voxels given the way
+// the reference passes one: its 12 Schmid tensors in the sample frame (AoS,
+// 60 per voxel) and its 12 CRSS; C holds the material, the strain rate and
+// L0.  Sample-frame arithmetic in the oracle's operation order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load_schmid_aos(const double* __restrict__ schmid, long v, double m[kSlips][5]) {
+#pragma unroll
+  for (int k = 0; k < kSlips; k++)
+#pragma unroll
+    for (int i = 0; i < 5; i++) m[k][i] = schmid[60 * v + 5 * k + i];
+}
+
+template <int OP>
+__global__ __launch_bounds__(kPointThreads) void k_solvers(
+    long n, const double* __restrict__ schmid, const double* __restrict__ anti, const double* __restrict__ crss,
+    const StepConst* __restrict__ Cp, const double* __restrict__ a5, const double* __restrict__ b5,
+    const double* __restrict__ c5, double eps, double max_rss, int max_it, double dt, double* __restrict__ acc,
+    double* __restrict__ err, double* __restrict__ out) {
+  const StepConst& C = *Cp;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
+    double m[kSlips][5], tau[kSlips], x[5];
+    load_schmid_aos(schmid, v, m);
+#pragma unroll
+    for (int k = 0; k < kSlips; k++) tau[k] = crss[12 * v + k];
+    switch (OP) {
+      case SOLVER_TAYLOR: {  // SolveConstitutiveEquations: a5 = initial stress
+        double s0[5];
+#pragma unroll
+        for (int i = 0; i < 5; i++) s0[i] = a5[5 * v + i];
+        faithful::taylor_nr(s0, m, tau, C, eps, max_it, x);
+#pragma unroll
+        for (int i = 0; i < 5; i++) out[5 * v + i] = x[i];
+        break;
+      }
+      case SOLVER_CONSTRAINED: {  // a5 = initial stress, b5 = lambda, c5 = d~
+        double s0[5], l[5], d[5];
+#pragma unroll
+        for (int i = 0; i < 5; i++) { s0[i] = a5[5 * v + i]; l[i] = b5[5 * v + i]; d[i] = c5[5 * v + i]; }
+        double e = err[v];
+        faithful::constrained_literal(s0, m, tau, C, l, d, eps, max_rss, max_it, x, &e);
+        err[v] = e;
+#pragma unroll
+        for (int i = 0; i < 5; i++) out[5 * v + i] = x[i];
+        break;
+      }
+      case SOLVER_SACHS: {  // SolveSachEquation (Solvers.cpp:230-243)
+        faithful::sachs(m, tau, C.edot, x);
+#pragma unroll
+        for (int i = 0; i < 5; i++) out[5 * v + i] = x[i];
+        break;
+      }
+      case SOLVER_APPLY: {  // ApplyConstitutiveEquations (:255-274): a5 = stress
+#pragma clang fp contract(off)
+        double s[5], e[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < 5; i++) s[i] = a5[5 * v + i];
+#pragma unroll
+        for (int k = 0; k < kSlips; k++) {
+          const double rss = faithful::dot5(m[k], s) / tau[k];
+          const double g = faithful::slip_rate(C, k, rss), sg = sgn(rss);
+#pragma unroll
+          for (int i = 0; i < 5; i++) e[i] += m[k][i] * g * sg;
+        }
+#pragma unroll
+        for (int i = 0; i < 5; i++) out[5 * v + i] = e[i];
+        break;
+      }
+      case SOLVER_SHEAR_RATE: {  // CalculateShearRate (:282-302): a5 = stress; out 9 per voxel
+#pragma clang fp contract(off)
+        double s[5], W[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 5; i++) s[i] = a5[5 * v + i];
+#pragma unroll
+        for (int k = 0; k < kSlips; k++) {
+          const double rss = faithful::dot5(m[k], s) / tau[k];
+          const double g = faithful::slip_rate(C, k, rss), sg = sgn(rss);
+#pragma unroll
+          for (int q = 0; q < 9; q++) W[q] += anti[108 * v + 9 * k + q] * g * sg;
+        }
+#pragma unroll
+        for (int q = 0; q < 9; q++) out[9 * v + q] = W[q];
+        break;
+      }
+      case SOLVER_HARDENING: {  // CalculateHardening (:314-373): a5 = stress, acc in/out; out 12 per voxel
+#pragma clang fp contract(off)
+        double s[5], g[kSlips], local = 0.0;
+#pragma unroll
+        for (int i = 0; i < 5; i++) s[i] = a5[5 * v + i];
+#pragma unroll
+        for (int k = 0; k < kSlips; k++) {
+          g[k] = faithful::slip_rate(C, k, faithful::dot5(m[k], s) / tau[k]);
+          local += fabs(g[k]) * dt;
+        }
+        const double A = acc[v];
+#pragma unroll
+        for (int i = 0; i < kSlips; i++) {
+          double dtau = 0.0;
+#pragma unroll
+          for (int j = 0; j < kSlips; j++) dtau += C.hmat[i * kSlips + j] * fabs(g[j]) * dt;
+          const double tiny = 1e-4 * C.tau0[i];
+          double voce = 0.0;
+          if (fabs(C.theta0[i]) > tiny) {
+            voce = C.theta1[i] * local;
+            if (fabs(C.tau1[i]) > tiny) {
+              const double fr = fabs(C.theta0[i] / C.tau1[i]);
+              const double e_ini = exp(-A * fr);
+              const double e_del = exp(-local * fr);
+              voce = voce - (fr * C.tau1[i] * C.theta1[i]) / fr * e_ini * (e_del - 1.0) -
+                     C.theta1[i] / fr * e_ini * (e_del * ((A + local) * fr + 1.0) - (A * fr + 1.0));
+            }
+          }
+          out[12 * v + i] = tau[i] + dtau * voce / local;
+        }
+        acc[v] = A + local;
+        break;
+      }
+    }
+  }
+}
+
+hipError_t launch_solvers(int op, long n, const double* schmid, const double* anti, const double* crss,
+                          const StepConst* C, const double* a5, const double* b5, const double* c5, double eps,
+                          double max_rss, int max_it, double dt, double* acc, double* err, double* out,
+                          hipStream_t st) {
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(point_blocks(n)), dim3(kPointThreads), 0, st, n, schmid, anti, crss, C, a5, b5,
+                       c5, eps, max_rss, max_it, dt, acc, err, out);
+    return hipGetLastError();
+  };
+  switch (op) {
+    case SOLVER_TAYLOR: return go(k_solvers<SOLVER_TAYLOR>);
+    case SOLVER_CONSTRAINED: return go(k_solvers<SOLVER_CONSTRAINED>);
+    case SOLVER_SACHS: return go(k_solvers<SOLVER_SACHS>);
+    case SOLVER_APPLY: return go(k_solvers<SOLVER_APPLY>);
+    case SOLVER_SHEAR_RATE: return go(k_solvers<SOLVER_SHEAR_RATE>);
+    case SOLVER_HARDENING: return go(k_solvers<SOLVER_HARDENING>);
+  }
+  return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------
