@@ -44,6 +44,9 @@ NR_FLOPS_PER_STEP = 570.0     # per Newton step
 # lambda, phi written (40 B each): the uniform-CRSS variant the workload runs
 # (every voxel's CRSS equal and kept equal by the hardening law)
 NR_BYTES = 320.0
+# the fused inverse-z / Newton / forward-z kernel (Nz = 256): the Newton
+# kernel's bytes without d~ and phi, plus the half spectrum read and written
+# (5 components x (Nz/2 + 1) complex bins per z row, each way)
 # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a
 # plain copy of known size) for the default workload
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "strain_step_256cubed_hbm_traffic.md")
@@ -335,7 +338,7 @@ def main():
 
     # roofline of the dominant kernel, from the HIP-event times of the same steps
     kt = f64["ktimes"]
-    loop_kernels = ("nr_update", "fft_z_r2c", "fft_y_fwd", "fft_x_green", "fft_y_inv", "fft_z_c2r")
+    loop_kernels = ("nr_update", "z_nr_fused", "fft_z_r2c", "fft_y_fwd", "fft_x_green", "fft_y_inv", "fft_z_c2r")
     dom = max(loop_kernels, key=lambda k: kt.get(k, (0.0, 0))[0])
     nc = args.size * args.size * (args.size // 2 + 1)
     nr_steps_total = sum(i["nr_steps_total"] for i in f64["infos"])
@@ -347,18 +350,20 @@ def main():
         if k.startswith("fft"):
             b = fft_bytes(k, n, nc, 8)
             entry["GBps"] = b / (ms / cnt) / 1e6
-        if k == "nr_update":
+        if k in ("nr_update", "z_nr_fused"):
             # per launch (one fixed-point iteration over the grid): the fixed
             # work of every voxel plus the Newton steps of an average iteration
+            # (the fused kernel's z transforms are not counted)
             flops = n * NR_FLOPS_FIXED + nr_steps_total / f64["iters"] * NR_FLOPS_PER_STEP
             entry["flops_per_launch"] = flops
             entry["TFLOPs"] = flops / (ms / cnt / 1e3) / 1e12
-            entry["GBps"] = NR_BYTES * n / (ms / cnt) / 1e6
+            b = NR_BYTES * n if k == "nr_update" else (NR_BYTES - 80.0) * n + 2 * 5 * nc * 16
+            entry["GBps"] = b / (ms / cnt) / 1e6
         per_kernel[k] = entry
-    if dom == "nr_update":
+    if dom in ("nr_update", "z_nr_fused"):
         ach = per_kernel[dom]["TFLOPs"]
         roof = {"bound": "fp64_valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / FP64_PEAK_TFLOPS, "traffic": measured_traffic(args, dom), "kernel": "nr_update",
+                "frac": ach / FP64_PEAK_TFLOPS, "traffic": measured_traffic(args, dom), "kernel": dom,
                 "note": "FP64 vector ALU work (no MFMA): MI355X's FP64 vector and matrix peaks are both "
                         "78.6 TFLOP/s"}
     else:

@@ -377,3 +377,60 @@ def test_laminate_n1_fixed_point_closed_form(gpu, vpfft, oracle):
     want = np.where(layerA[:, None], sA, sB)
     assert rel_err(sigma, want) < 1e-10, rel_err(sigma, want)
     assert rel_err(avg, fA * sA + (1 - fA) * sB) < 1e-10
+
+
+def _fused_run(vpfft, dims, R, mat, edot, prec, fuse, graph_iters=None, profile=False, steps=2):
+    """Two strain steps; returns per-step <sigma> and iterations and the final
+    fields.  VPFFT_FUSE_Z and VPFFT_GRAPH_ITERS are read at grid creation."""
+    import os
+    env = {"VPFFT_FUSE_Z": "1" if fuse else "0"}
+    if graph_iters is not None:
+        env["VPFFT_GRAPH_ITERS"] = str(graph_iters)
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        g = vpfft.MaterialGrid(*dims, mat, crss=1.0, precision=prec)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    with g:
+        g.SetVoxelLength(1.0, 1.0, 1.0)
+        g.SendDataAndInitialize(R)
+        if profile:
+            g.SetProfiling(True)
+        out = []
+        for _ in range(steps):
+            out.append((g.RunSingleStrainStep(edot, 100, 0.005, 1e-6), g.last_info.iterations))
+        fields = [g.Stress(), g.LagrangeMultiplier(), g.PolarizationField(), g.DisplacementVariation(),
+                  g.Orientation(), g.CRSS()]
+        kt = g.KernelTimes() if profile else None
+    return out, fields, kt
+
+
+@pytest.mark.parametrize("prec,graph_iters,profile", [(0, None, False), (1, None, False), (0, 3, False),
+                                                      (0, None, True)])
+def test_fused_z_loop_matches_unfused(gpu, vpfft, prec, graph_iters, profile):
+    """Nz = 256: the fused inverse-z / Newton / forward-z kernel gives the bits
+    of the three separate kernels -- every step's <sigma> and
+    iteration count, and the final stress, Lagrange multiplier, polarization
+    and strain-rate variation (both rebuilt after the loop), orientation and
+    CRSS -- graph-replayed (graph length 4, and 3, which a fused loop rounds
+    up to an even length) and host-launched."""
+    from helpers import polycrystal
+    dims = (8, 10, 256)
+    print("fused z loop", dims, "prec", prec, "graph", graph_iters, "profile", profile, flush=True)
+    R, _ = polycrystal(0, 24, 5, dims=dims)
+    mat = vpfft.Material.uniform(1.0, 20)
+    edot = random_unit_edot(21)
+    a, fa, kt = _fused_run(vpfft, dims, R, mat, edot, prec, True, graph_iters, profile)
+    b, fb, _ = _fused_run(vpfft, dims, R, mat, edot, prec, False, graph_iters, profile)
+    for (sa, ia), (sb, ib) in zip(a, b):
+        assert ia == ib
+        assert np.array_equal(sa, sb)
+    for x, y in zip(fa, fb):
+        assert np.array_equal(x, y)
+    if profile:
+        assert kt["z_nr_fused"][1] > 0 and kt["fft_z_c2r"][1] == 0 and kt["nr_update"][1] == 0

@@ -91,11 +91,11 @@ static int fill_blocks(long n) {
 
 enum KernelId {
   K_PROLOGUE = 0, K_L0_FINALIZE, K_Z_R2C, K_Y_FWD, K_X_GREEN, K_Y_INV, K_Z_C2R, K_NR, K_ITER_FINALIZE,
-  K_EPILOGUE, kNumKernels
+  K_EPILOGUE, K_ZNR, kNumKernels
 };
 static const char* kKernelNames[kNumKernels] = {"prologue", "L0_finalize", "fft_z_r2c", "fft_y_fwd",
                                                 "fft_x_green", "fft_y_inv", "fft_z_c2r", "nr_update",
-                                                "iter_finalize", "epilogue"};
+                                                "iter_finalize", "epilogue", "z_nr_fused"};
 
 struct Grid {
   unsigned long serial = 0;  // unique per created grid (keys of captured batch graphs)
@@ -107,6 +107,12 @@ struct Grid {
   hipStream_t st = nullptr;
   double *R = nullptr, *crss = nullptr, *acc = nullptr, *sigma = nullptr, *lambda = nullptr;
   void *phi = nullptr, *dtil = nullptr, *spec = nullptr;  // sized for FP64
+  // Nz = 256: the loop runs the fused inverse-z / Newton / forward-z kernel,
+  // alternating the half spectrum between spec and spec2 (VPFFT_FUSE_Z=0
+  // turns it off; the separate kernels then take the fused kernel's runs of
+  // 64 voxels, so both forms give the same bits)
+  void* spec2 = nullptr;
+  bool fuse_z = false;
   void* tw64[3] = {nullptr, nullptr, nullptr};
   void* tw32[3] = {nullptr, nullptr, nullptr};
   AxisPlan plan[3];
@@ -398,7 +404,8 @@ static int check_material(const vpfft_material* m) {
 static size_t grid_bytes(const vpfft_grid_desc* d) {
   const long n = (long)d->nx * d->ny * d->nz;
   const long nc = (long)d->nx * d->ny * padded_half(d->nz);
-  size_t b = sizeof(double) * 42 * (size_t)n + sizeof(double2) * 5 * (size_t)nc;
+  const bool fused = d->nz == 256;
+  size_t b = sizeof(double) * 42 * (size_t)n + sizeof(double2) * 5 * (size_t)nc * (fused ? 2 : 1);
   b += sizeof(StepConst) + sizeof(GreenConst<double>) + sizeof(GreenConst<float>) + sizeof(PrologueRecord) +
        sizeof(HealthRecord) + sizeof(LoopState) + sizeof(IterRecord) * 128;
   b += sizeof(double) * (size_t)std::max<long>(2L * 4096 * 18, nr_partial_rows(n) * kPartialStride);
@@ -494,6 +501,11 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
   ALLOC(g->phi, sizeof(double) * 5 * n);
   ALLOC(g->dtil, sizeof(double) * 5 * n);
   ALLOC(g->spec, sizeof(double2) * 5 * g->nc);
+  if (g->nz == 256) {
+    ALLOC(g->spec2, sizeof(double2) * 5 * g->nc);
+    const char* f = std::getenv("VPFFT_FUSE_Z");
+    g->fuse_z = !(f && std::atoi(f) == 0);
+  }
   ALLOC(g->dC, sizeof(StepConst));
   ALLOC(g->dG64, sizeof(GreenConst<double>));
   ALLOC(g->dG32, sizeof(GreenConst<float>));
@@ -523,14 +535,46 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
 // One fixed-point iteration: FFT + Green + inverse FFT, the Newton kernel,
 // and the device-side convergence test.  Every kernel returns at entry once
 // the loop state says done, so the same sequence can be captured in a graph.
-static int enqueue_iteration(Grid* g, int prec, double tol, int max_iter) {
+// The fused loop applies to this step: the grid is eligible and its z pass
+// uses the 256-point register form (the default block shape).
+static bool fused_step(const Grid* g, int prec) { return g->fuse_z && g->fftB[prec][0] == 0; }
+
+// Iteration k of a fused loop: y, x + Green, y inverse on the half spectrum
+// the previous iteration left (spec for even k, spec2 for odd k), then the
+// fused kernel writes the next half spectrum to the other buffer.
+static void fused_buffers(Grid* g, int k, void** in, void** out) {
+  *in = (k & 1) ? g->spec2 : g->spec;
+  *out = (k & 1) ? g->spec : g->spec2;
+}
+
+static int enqueue_iteration(Grid* g, int prec, double tol, int max_iter, int k) {
   const int* done = &g->dls->done;
   int rc;
-  if ((rc = run_fft_green(g, prec, done))) return rc;
-  kbegin(g, K_NR);
-  HIP_TRY(launch_nr(prec, g->nexp_uniform, g->crss_uniform, g->n, g->dtil, g->phi, g->sigma, g->lambda, g->R, g->crss,
-                    g->dC, g->partial, g->dls, g->dhealth, g->st));
-  kend(g, K_NR);
+  if (fused_step(g, prec)) {
+    void* const* tw = prec == 0 ? g->tw64 : g->tw32;
+    const void* gc = prec == 0 ? (const void*)g->dG64 : (const void*)g->dG32;
+    void *sin, *sout;
+    fused_buffers(g, k, &sin, &sout);
+    kbegin(g, K_Y_FWD);
+    HIP_TRY(launch_y(prec, sin, g->geom, g->plan[1], tw[1], -1, g->fftB[prec][1], done, g->st));
+    kend(g, K_Y_FWD);
+    kbegin(g, K_X_GREEN);
+    HIP_TRY(launch_x_green(prec, sin, g->geom, g->plan[0], tw[0], gc, g->fftB[prec][2], done, g->st));
+    kend(g, K_X_GREEN);
+    kbegin(g, K_Y_INV);
+    HIP_TRY(launch_y(prec, sin, g->geom, g->plan[1], tw[1], +1, g->fftB[prec][1], done, g->st));
+    kend(g, K_Y_INV);
+    kbegin(g, K_ZNR);
+    HIP_TRY(launch_zfused(prec, g->nexp_uniform, g->crss_uniform, g->n, sin, sout, g->geom, tw[2], 1.0 / (double)g->n,
+                          g->sigma, g->lambda, g->R, g->crss, g->dC, g->partial, g->dls, g->dhealth, g->st));
+    kend(g, K_ZNR);
+  } else {
+    if ((rc = run_fft_green(g, prec, done))) return rc;
+    kbegin(g, K_NR);
+    HIP_TRY(launch_nr(prec, g->nexp_uniform, g->crss_uniform, g->n, g->dtil, g->phi, g->sigma, g->lambda, g->R,
+                      g->crss, g->dC, g->partial, g->dls, g->dhealth, g->nz == 256, g->st));
+    kend(g, K_NR);
+  }
   g->phi_prec = prec;
   kbegin(g, K_ITER_FINALIZE);
   HIP_TRY(launch_iter_finalize(g->n, g->partial, g->dC, tol, max_iter, g->dls, g->drec, g->st));
@@ -616,11 +660,21 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
     return set_error(VPFFT_E_NONFINITE, "reference medium L0 (or its inverse) is not symmetric positive definite");
   // fixed-point loop: MaterialGrid.cpp:930-1068
   HIP_TRY(launch_loop_reset(g->dls, g->st));
+  const bool fused = fused_step(g, prec);
+  if (fused) {  // the fused loop starts from the half spectrum of the prologue's phi
+    void* const* tw = prec == 0 ? g->tw64 : g->tw32;
+    kbegin(g, K_Z_R2C);
+    HIP_TRY(launch_z_r2c(prec, g->phi, g->spec, g->geom, g->plan[2], tw[2], 0, nullptr, g->st));
+    kend(g, K_Z_R2C);
+  }
   int it = 0;
   double nr_total = 0, wave_total = 0;
   const bool use_graph = g->graph_iters > 0 && !g->profile;
+  // a fused loop alternates two spectrum buffers, so a replayed graph holds an
+  // even number of iterations
+  const int graph_iters = fused ? (g->graph_iters + 1) / 2 * 2 : g->graph_iters;
   if (use_graph) {
-    const Grid::GraphKey key{prec, max_iter, g->nexp_uniform, g->crss_uniform ? 1 : 0, tol};
+    const Grid::GraphKey key{prec, max_iter, g->nexp_uniform, (g->crss_uniform ? 1 : 0) | (fused ? 2 : 0), tol};
     if (!g->gexec || !(g->gkey == key)) {
       if (g->gexec) {
         HIP_TRY(hipGraphExecDestroy(g->gexec));
@@ -629,7 +683,7 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
       hipGraph_t graph;
       HIP_TRY(hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal));
       int crc = VPFFT_OK;
-      for (int k = 0; k < g->graph_iters && crc == VPFFT_OK; k++) crc = enqueue_iteration(g, prec, tol, max_iter);
+      for (int k = 0; k < graph_iters && crc == VPFFT_OK; k++) crc = enqueue_iteration(g, prec, tol, max_iter, k);
       hipError_t ce = hipStreamEndCapture(g->st, &graph);
       if (crc) return crc;
       HIP_TRY(ce);
@@ -656,7 +710,7 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
     }
   } else {
     for (;;) {
-      if ((rc = enqueue_iteration(g, prec, tol, max_iter))) return rc;
+      if ((rc = enqueue_iteration(g, prec, tol, max_iter, it))) return rc;
       HIP_TRY(hipMemcpyAsync(&g->hrec[it], &g->drec[it], sizeof(IterRecord), hipMemcpyDeviceToHost, g->st));
       HIP_TRY(hipStreamSynchronize(g->st));
       kcollect(g);
@@ -669,6 +723,17 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
       it++;
       if (r.done) break;
     }
+  }
+  if (fused) {
+    // d~ and phi of the last iteration lived in LDS only: rebuild them from
+    // the half spectrum that iteration read (the same inverse transform) and
+    // from the final stress (the same polarization)
+    void *sin, *sout;
+    fused_buffers(g, it - 1, &sin, &sout);
+    void* const* tw = prec == 0 ? g->tw64 : g->tw32;
+    HIP_TRY(launch_z_c2r(prec, sin, g->dtil, g->geom, g->plan[2], tw[2], 1.0 / (double)g->n, 0, nullptr, g->st));
+    HIP_TRY(launch_phi_rebuild(prec, g->n, g->dtil, g->phi, g->sigma, g->dC, g->st));
+    g->dtil_prec = prec;
   }
   HIP_TRY(hipEventRecord(g->ev[2], g->st));
   const IterRecord& last = g->hrec[it - 1];

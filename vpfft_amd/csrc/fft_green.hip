@@ -29,6 +29,7 @@
 
 #include "device_common.h"
 #include "fft_green.h"
+#include "fft_regs.h"
 
 // No multiply-add contraction in this file: with it the compiler's choice of
 // which products to fuse can differ between the block-shape instantiations
@@ -37,36 +38,6 @@
 #pragma STDC FP_CONTRACT OFF
 
 namespace vpfft {
-
-template <typename T> struct C2;
-template <> struct C2<double> { using t = double2; };
-template <> struct C2<float> { using t = float2; };
-
-template <typename T> __device__ __forceinline__ typename C2<T>::t cmake(T a, T b) {
-  typename C2<T>::t r;
-  r.x = a;
-  r.y = b;
-  return r;
-}
-template <typename C> __device__ __forceinline__ C cadd(C a, C b) { C r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
-template <typename C> __device__ __forceinline__ C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
-// fused multiply-adds written out (contraction is off in this file)
-__device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-
-template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
-  C r;
-  r.x = fmad(a.x, b.x, -(a.y * b.y));
-  r.y = fmad(a.x, b.y, a.y * b.x);
-  return r;
-}
-// multiply by (s * i), s = +-1
-template <typename C> __device__ __forceinline__ C cmul_si(C a, int s) {
-  C r;
-  r.x = s > 0 ? -a.y : a.y;
-  r.y = s > 0 ? a.x : -a.x;
-  return r;
-}
 
 // cos / sin (2 pi t / R) for the odd radices
 __device__ __constant__ static const double kCos3[3] = {1.0, -0.5, -0.5};
@@ -113,12 +84,7 @@ __device__ __forceinline__ void dft(typename C2<T>::t* v, int sign) {
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
   } else if constexpr (R == 4) {
-    C a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
-    C c = cadd(v[1], v[3]), d = cmul_si(csub(v[1], v[3]), sign);
-    v[0] = cadd(a, c);
-    v[2] = csub(a, c);
-    v[1] = cadd(b, d);
-    v[3] = csub(b, d);
+    dft4<T>(v, sign);
   } else if constexpr (R == 8) {
     C e[4] = {v[0], v[2], v[4], v[6]};
     C o[4] = {v[1], v[3], v[5], v[7]};
@@ -396,42 +362,6 @@ __global__ void __launch_bounds__(256) k_y(typename C2<T>::t* __restrict__ s, Ff
 // its k1; a second 16-point DFT yields X[k1 + 16 k2].  One LDS round trip
 // replaces the three of the generic Stockham passes.
 // ---------------------------------------------------------------------------
-// 16-point DFT of v in place (natural order), W = exp(sign 2 pi i / 16):
-// 4 DFT-4 over m of v[a + 4m], twiddles W^(a b), 4 DFT-4 over a.
-template <typename T>
-__device__ __forceinline__ void dft16(typename C2<T>::t v[16], int sign) {
-  using C = typename C2<T>::t;
-  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, h = (T)0.70710678118654752440;
-  C u[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
-    C w[4] = {v[a], v[a + 4], v[a + 8], v[a + 12]};
-    dft<T, 4>(w, sign);
-#pragma unroll
-    for (int b = 0; b < 4; b++) u[a][b] = w[b];
-  }
-  // W16^(ab) for ab = 1, 2, 3, 4, 6, 9 (a, b in 1..3)
-  const T sg = (T)sign;
-  const C w1 = cmake<T>(c1, sg * s1), w2 = cmake<T>(h, sg * h), w3 = cmake<T>(s1, sg * c1);
-  const C w6 = cmake<T>(-h, sg * h), w9 = cmake<T>(-c1, -sg * s1);
-  u[1][1] = cmul(u[1][1], w1);
-  u[1][2] = cmul(u[1][2], w2);
-  u[1][3] = cmul(u[1][3], w3);
-  u[2][1] = cmul(u[2][1], w2);
-  u[2][2] = cmul_si(u[2][2], sign);  // W^4 = sign i
-  u[2][3] = cmul(u[2][3], w6);
-  u[3][1] = cmul(u[3][1], w3);
-  u[3][2] = cmul(u[3][2], w6);
-  u[3][3] = cmul(u[3][3], w9);
-#pragma unroll
-  for (int b = 0; b < 4; b++) {
-    C w[4] = {u[0][b], u[1][b], u[2][b], u[3][b]};
-    dft<T, 4>(w, sign);
-#pragma unroll
-    for (int c = 0; c < 4; c++) v[b + 4 * c] = w[c];
-  }
-}
-
 // Exchange through LDS: thread (p, t) holds V[k1] (k1 = 0..15) of pencil p;
 // afterwards thread (p, t) holds W[tt] = V_tt[t] for tt = 0..15.  buf: 4096
 // reals, laid out [k1][t][p] with p fastest so stores and loads are 128-byte
@@ -707,23 +637,6 @@ __global__ void __launch_bounds__(512) k_z_c2r512(const typename C2<T>::t* __res
 // pencil is the fastest index so a wave reads 128-byte runs along z.
 // LDS rows are padded to 257 reals so the transposing reads hit distinct banks.
 template <typename T>
-__device__ __forceinline__ void exchange16_zt(typename C2<T>::t v[16], T* buf, int p, int t) {
-  T re[16];
-#pragma unroll
-  for (int k = 0; k < 16; k++) buf[k * 257 + p * 16 + t] = v[k].x;
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < 16; k++) re[k] = buf[t * 257 + p * 16 + k];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < 16; k++) buf[k * 257 + p * 16 + t] = v[k].y;
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < 16; k++) v[k] = cmake<T>(re[k], buf[t * 257 + p * 16 + k]);
-  __syncthreads();
-}
-
-template <typename T>
 __global__ void __launch_bounds__(256) k_z_r2c256(const T* __restrict__ f, typename C2<T>::t* __restrict__ s,
                                                   FftGeom g, const typename C2<T>::t* __restrict__ tw,
                                                   const int* __restrict__ done) {
@@ -744,41 +657,9 @@ __global__ void __launch_bounds__(256) k_z_r2c256(const T* __restrict__ f, typen
     const T b = (active && pair < 2) ? fb[t + 16 * j] : (T)0;
     v[j] = cmake<T>(a, b);
   }
-  dft16<T>(v, -1);
-#pragma unroll
-  for (int k = 1; k < 16; k++) v[k] = cmul(v[k], tw[t * k]);
-  exchange16_zt<T>(v, buf, p, t);
-  dft16<T>(v, -1);  // thread t now holds Z[t + 16 k2]
-  // Hermitian unpack: X_a = (Z_k + conj Z_-k) / 2, X_b = -i/2 (Z_k - conj Z_-k)
-  // for k = t + 16 j <= 128; Z in natural order through LDS, real parts first
-  T zr[9], zi[9], nr[9], ni[9];
-#pragma unroll
-  for (int ph = 0; ph < 2; ph++) {
-#pragma unroll
-    for (int k2 = 0; k2 < 16; k2++) buf[p * 257 + t + 16 * k2] = ph == 0 ? v[k2].x : v[k2].y;
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < 9; j++) {
-      const int k = t + 16 * j;
-      const int kn = (256 - k) & 255;
-      const T a = k <= 128 ? buf[p * 257 + k] : (T)0;
-      const T b = k <= 128 ? buf[p * 257 + kn] : (T)0;
-      if (ph == 0) { zr[j] = a; nr[j] = b; } else { zi[j] = a; ni[j] = b; }
-    }
-    __syncthreads();
-  }
-  if (active) {
-    C* sa = s + (long)(2 * pair) * g.nc + row * g.nzp;
-    C* sb = s + (long)(2 * pair + 1) * g.nc + row * g.nzp;
-    const T h = (T)0.5;
-#pragma unroll
-    for (int j = 0; j < 9; j++) {
-      const int k = t + 16 * j;
-      if (k > 128) continue;
-      sa[k] = cmake<T>(h * (zr[j] + nr[j]), h * (zi[j] - ni[j]));
-      if (pair < 2) sb[k] = cmake<T>(h * (zi[j] + ni[j]), -h * (zr[j] - nr[j]));
-    }
-  }
+  C* sa = s + (long)(2 * pair) * g.nc + row * g.nzp;
+  C* sb = pair < 2 ? s + (long)(2 * pair + 1) * g.nc + row * g.nzp : nullptr;
+  z_r2c256_pencil<T>(v, active, tw, buf, p, t, sa, sb);
 }
 
 template <typename T>
@@ -794,33 +675,9 @@ __global__ void __launch_bounds__(256) k_z_c2r256(const typename C2<T>::t* __res
   const int pair = (int)(P - row * 3);
   const bool active = row < g.nrows;
   const C* sa = s + (long)(2 * pair) * g.nc + row * g.nzp;
-  const C* sb = s + (long)(2 * pair + 1) * g.nc + row * g.nzp;
+  const C* sb = pair < 2 ? s + (long)(2 * pair + 1) * g.nc + row * g.nzp : nullptr;
   C v[16];
-#pragma unroll
-  for (int j = 0; j < 16; j++) {
-    const int k = t + 16 * j;
-    const int kk = k <= 128 ? k : 256 - k;  // Z_-k from the conjugates
-    C xa = cmake<T>(0, 0), xb = cmake<T>(0, 0);
-    if (active) {
-      xa = sa[kk];
-      if (pair < 2) xb = sb[kk];
-    }
-    if (kk == 0 || kk == 128) {  // C2R reads only the real part of the self-conjugate bins
-      xa.y = 0;
-      xb.y = 0;
-    }
-    // Z_k = X_a + i X_b ;  Z_-k = conj X_a + i conj X_b
-    v[j] = k <= 128 ? cmake<T>(xa.x - xb.y, xa.y + xb.x) : cmake<T>(xa.x + xb.y, xb.x - xa.y);
-  }
-  dft16<T>(v, +1);
-#pragma unroll
-  for (int k = 1; k < 16; k++) {
-    C w = tw[t * k];
-    w.y = -w.y;
-    v[k] = cmul(v[k], w);
-  }
-  exchange16_zt<T>(v, buf, p, t);
-  dft16<T>(v, +1);  // thread t holds z = t + 16 k2
+  z_c2r256_pencil<T>(sa, sb, active, tw, buf, p, t, v);
   if (active) {
     T* fa = f + (long)(2 * pair) * g.n + row * 256;
     T* fb = f + (long)(2 * pair + 1) * g.n + row * 256;

@@ -95,10 +95,12 @@ hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenCon
 // waves: 1 or 2 waves per SIMD (register budget 512 or 256 per lane);
 // uniform_tau: every slip system of a voxel has the same CRSS (crss[0][v] is read)
 // ls: the loop state (kernels return at entry once ls->done is set; may be
-// null outside a loop); health: where the first non-finite value is recorded
+// null outside a loop); health: where the first non-finite value is recorded;
+// linear: each wave takes 64 consecutive voxels (the fused kernel's runs)
+// instead of a 4 x 16 tile
 hipError_t launch_nr(int prec, int nexp_uniform, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
                      double* lambda, const double* R, const double* crss, const StepConst* C, double* partial,
-                     const LoopState* ls, HealthRecord* health, hipStream_t st);
+                     const LoopState* ls, HealthRecord* health, bool linear, hipStream_t st);
 hipError_t launch_iter_finalize(long n, const double* partial, const StepConst* C, double tol,
                                 int max_iter, LoopState* ls, IterRecord* rec, hipStream_t st);
 hipError_t launch_loop_reset(LoopState* ls, hipStream_t st);
@@ -116,6 +118,18 @@ hipError_t launch_constrained_batch(long n, const double* R, const double* crss,
                                     double* err, int* steps, hipStream_t st);
 hipError_t launch_taylor_batch(long n, const double* R, const double* crss, const StepConst* C, const double* s_in,
                                double* s_out, hipStream_t st);
+
+// The fused inverse-z / per-voxel body / forward-z kernel of one iteration
+// (Nz = 256), and the rebuild of phi = sigma - L0 d~ after a fused loop.  The fused kernel reads the half spectrum spec_in and writes
+// spec_out (different buffers), fills the same partial rows as launch_nr and
+// folds them the same way.
+bool zfused_eligible(const FftGeom& g);
+hipError_t launch_zfused(int prec, int nexp_uniform, bool uniform_tau, long n, const void* spec_in, void* spec_out,
+                         const FftGeom& g, const void* tw, double scale, double* sigma, double* lambda,
+                         const double* R, const double* crss, const StepConst* C, double* partial,
+                         const LoopState* ls, HealthRecord* health, hipStream_t st);
+hipError_t launch_phi_rebuild(int prec, long n, const void* dtil, void* phi, const double* sigma, const StepConst* C,
+                              hipStream_t st);
 
 // VPFFT::Solvers over n voxels given by their sample-frame Schmid tensors
 // (n x 60) and CRSS (n x 12); C: material, strain rate, L0.  a5/b5/c5 are

@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
+#include "fft_green.h"
+#include "fft_regs.h"
 #include "voxel_kernels.h"
 
 namespace vpfft {
@@ -492,14 +494,41 @@ __device__ __forceinline__ VoxJob vox_job(const VoxJob* J) {
 // per-wave max NR steps; then max NR error, max NR steps
 // NE > 0: every slip system has rate-sensitivity exponent NE.
 // ---------------------------------------------------------------------------
-// BATCHED: the grid's pointers come from the job table of grp (blocks of
-// many grids in one launch); else from the kernel arguments.
-template <typename T, int NE, bool UNIFORM_TAU, bool BATCHED>
+// MODE: NR_SINGLE takes the grid's pointers from the kernel arguments and
+// d~ / phi from HBM; NR_BATCHED the pointers from the job table of grp
+// (blocks of many grids in one launch); NR_ZFUSED (Nz = 256) is one
+// fixed-point iteration's inverse z transform, this body and the forward z
+// transform in one kernel: a block (one wave) takes one z row of all five
+// components; its 3 pencils (component pairs) of the half spectrum
+// zf.spec_in go through the inverse transforms into LDS, the wave runs the
+// body on the row's 4 runs of 64 voxels, and the phi row goes through the
+// forward transforms to zf.spec_out (another buffer: the caller alternates
+// them, so d~ of the last iteration can be rebuilt after the loop).  Neither
+// real field crosses HBM, and each wave moves between its transforms and
+// its Newton solves on its own, so the memory-bound and the FP64-bound work
+// of different waves overlap on a CU.  The operations are those of
+// k_z_c2r256, the single form with zf.linear set (a block takes 64
+// consecutive voxels) and k_z_r2c256, so the bits are the same.
+enum NrMode { NR_SINGLE = 0, NR_BATCHED = 1, NR_ZFUSED = 2 };
+
+struct ZFusedArgs {
+  const void* spec_in;
+  void* spec_out;
+  const void* tw;  // W256^j in the transform precision
+  long nc;         // spectrum stride per component
+  int nzp;         // spectrum row stride
+  double scale;    // 1/N of the inverse transform
+  int linear;      // NR_SINGLE: blocks take 64 consecutive voxels (the fused form's runs)
+};
+
+template <typename T, int NE, bool UNIFORM_TAU, int MODE>
 __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu(kNrWaves, kNrWaves)))
-k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restrict__ sigma_,
-     double* __restrict__ lambda_, const double* __restrict__ R_, const double* __restrict__ crss_,
-     const StepConst* __restrict__ Cp_, double* __restrict__ partial_, const LoopState* __restrict__ ls_,
-     HealthRecord* __restrict__ health_, VoxGroup grp) {
+    k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restrict__ sigma_,
+         double* __restrict__ lambda_, const double* __restrict__ R_, const double* __restrict__ crss_,
+         const StepConst* __restrict__ Cp_, double* __restrict__ partial_, const LoopState* __restrict__ ls_,
+         HealthRecord* __restrict__ health_, VoxGroup grp, ZFusedArgs zf) {
+  constexpr bool FUSED = MODE == NR_ZFUSED;
+  using Cx = typename C2<T>::t;
   long n, blk;
   const T* dtil;
   T* phi;
@@ -508,7 +537,7 @@ k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restr
   const StepConst* Cp;
   const LoopState* ls;
   HealthRecord* health;
-  if constexpr (BATCHED) {
+  if constexpr (MODE == NR_BATCHED) {
     const VoxJob J = vox_job(grp.jobs + vox_member(grp, blockIdx.x, &blk));
     if (J.ls->done) return;
     ls = J.ls;
@@ -537,119 +566,187 @@ k_nr(long n_, const T* __restrict__ dtil_, T* __restrict__ phi_, double* __restr
     partial = partial_;
     blk = blockIdx.x;
   }
-  const StepConst& C = *as_const_mem(Cp);
-  double acc[15];
+  // the fused form's LDS: the exchange buffer of the z transforms and the
+  // row's d~ (then phi), [component][z]
+  __shared__ T zbuf[FUSED ? zbuf_reals<3>() : 1];
+  __shared__ T fld[FUSED ? 5 * 256 : 1];
+  const int zt16 = threadIdx.x & 15, zpair = threadIdx.x >> 4;
+  const bool zactive = zpair < 3;
+  const long zrow = blockIdx.x;
+  if constexpr (FUSED) {  // d~ row: inverse z transforms into LDS
+    const Cx* si = (const Cx*)zf.spec_in;
+    const Cx* sa = si + (long)(2 * zpair) * zf.nc + zrow * zf.nzp;
+    const Cx* sb = zpair < 2 ? si + (long)(2 * zpair + 1) * zf.nc + zrow * zf.nzp : nullptr;
+    Cx v[16];
+    z_c2r256_pencil<T, 3>(sa, sb, zactive, (const Cx*)zf.tw, zbuf, zpair, zt16, v);
+    if (zactive) {
 #pragma unroll
-  for (int w = 0; w < 15; w++) acc[w] = 0.0;
-  double mx[2] = {0.0, 0.0};  // max NR error, max NR steps
-  // one voxel per lane: nothing but the voxel's own state is live
-  const long v = tile_voxel<kNrThreads>(blk, threadIdx.x, C.ny, C.nz);
-  if (v < n) {
-    double it[kSlips], r9[9];
-    load_R(R, n, v, r9);
-    if (UNIFORM_TAU) {
-      // every slip system of the voxel has the same CRSS (one load)
-      const double t = 1.0 / crss[v];
-#pragma unroll
-      for (int k = 0; k < kSlips; k++) it[k] = t;
-    } else {
-      load_inv_tau(crss, n, v, it);
+      for (int k2 = 0; k2 < 16; k2++) {
+        fld[(2 * zpair) * 256 + zt16 + 16 * k2] = v[k2].x * (T)zf.scale;
+        if (zpair < 2) fld[(2 * zpair + 1) * 256 + zt16 + 16 * k2] = v[k2].y * (T)zf.scale;
+      }
     }
-    double s[5];
-#pragma unroll
-    for (int i = 0; i < 5; i++) s[i] = sigma[i * n + v];
-    // yield-surface rescale, MaterialGrid.cpp:954-959
-    double mrss, sc[5];
-    {
-      double Q[25];
-      cf::dev_map(r9, Q);
-      cf::rot(Q, s, sc);
-      mrss = cf::max_rss(sc, it);
-    }
-    if (!(mrss >= 1e-10)) acc[13] += 1.0;
-    const double imrss = rcp_nr(mrss);
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-      s[i] /= mrss;
-      sc[i] *= imrss;  // the crystal-frame start point of the solve
-    }
-    // q = L0^-1 lambda + edot + d~ ; lambda and d~ are re-read after the solve
-    double q[5];
-#pragma unroll
-    for (int i = 0; i < 5; i++) q[i] = C.edot[i] + ld(dtil + i * n + v);
-#pragma unroll
-    for (int j = 0; j < 5; j++) {
-      const double lj = lambda[j * n + v];
-#pragma unroll
-      for (int i = 0; i < 5; i++) q[i] += C.L0inv[j * 5 + i] * lj;
-    }
-    double err = 0.0;
-    int steps = 0;
-    {
-      double P[25], qc[5], A0[15];
-      cf::inverse_map(r9, P);
-      cf::rot_t(P, q, qc);
-      cf::congruence_t(P, C.L0inv, A0);
-      cf::constrained_nr<NE>(s, sc, r9, P, it, C, qc, A0, 5e-4, 1e9, 100, &err, &steps);
-    }
-    double e[5];
-    {
-      double Q[25], sc[5], ec[5];
-      cf::dev_map(r9, Q);
-      cf::rot(Q, s, sc);
-      cf::apply_constitutive<NE>(sc, it, C, ec);
-      cf::rot_t(Q, ec, e);
-    }
-    // re-read d~ and lambda here rather than keep them live through the solve
-    __asm__ volatile("" ::: "memory");
-    double d[5], lam[5];
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-      d[i] = ld(dtil + i * n + v);
-      lam[i] = lambda[i * n + v];
-    }
-    double Ld[5], t[5], Lt[5];
-    matvec5(C.L0, d, Ld);
-#pragma unroll
-    for (int i = 0; i < 5; i++) t[i] = C.edot[i] + d[i] - e[i];
-    matvec5(C.L0, t, Lt);
-    double se2 = 0.0, ee2 = 0.0;
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-      lam[i] += Lt[i];
-      phi[i * n + v] = (T)(s[i] - Ld[i]);
-      sigma[i * n + v] = s[i];
-      lambda[i * n + v] = lam[i];
-      const double se = s[i] - lam[i];
-      const double ee = e[i] - C.edot[i] - d[i];
-      se2 += se * se;
-      ee2 += ee * ee;
-      acc[i] += s[i];
-      acc[5 + i] += e[i];
-    }
-    acc[10] += sqrt(se2);
-    acc[11] += sqrt(ee2);
-    acc[12] += (double)steps;
-    mx[0] = fmax(mx[0], err);
-    mx[1] = fmax(mx[1], (double)steps);
-    // the first non-finite value of the step: the FFT's output, then what
-    // this kernel made of it
-    const double fd = sum5(d), fs = sum5(s), fe = sum5(e), fl = sum5(lam);
-    if (__builtin_expect(!__builtin_isfinite(fd + fs + fe + fl), 0)) {
-      const int f = !__builtin_isfinite(fd) ? HF_DTILDE
-                    : !__builtin_isfinite(fs) ? HF_STRESS
-                    : !__builtin_isfinite(fe) ? HF_STRAIN_RATE : HF_LAMBDA;
-      const double val = f == HF_DTILDE ? fd : f == HF_STRESS ? fs : f == HF_STRAIN_RATE ? fe : fl;
-      report_health(health, HK_NR, f, ls ? ls->iter + 1 : 0, v, val);
-    }
+    __syncthreads();
   }
-  // lane efficiency of the Newton loop: a wave runs as many steps as its
-  // slowest lane, the wave's maximum of the steps (slot 14 is 0 in every
-  // lane before the reduction and 64 x that maximum after it)
-  wave_sum<15>(acc, mx, partial + blk * kPartialStride);
-  if ((threadIdx.x & 63) == 63) partial[blk * kPartialStride + 14] = 64.0 * mx[1];
+  const StepConst& C = *as_const_mem(Cp);
+  for (int q = 0; q < (FUSED ? 4 : 1); q++) {
+    double acc[15];
+#pragma unroll
+    for (int w = 0; w < 15; w++) acc[w] = 0.0;
+    double mx[2] = {0.0, 0.0};  // max NR error, max NR steps
+    // one voxel per lane: nothing but the voxel's own state is live
+    long v, slot = 0;
+    if constexpr (FUSED) {
+      slot = q * 64 + (threadIdx.x & 63);
+      v = zrow * 256 + slot;
+      blk = zrow * 4 + q;
+    } else if (MODE == NR_SINGLE && zf.linear) {
+      v = blk * kNrThreads + threadIdx.x;
+    } else {
+      v = tile_voxel<kNrThreads>(blk, threadIdx.x, C.ny, C.nz);
+    }
+    if (v < n) {
+      double it[kSlips], r9[9];
+      load_R(R, n, v, r9);
+      if (UNIFORM_TAU) {
+        // every slip system of the voxel has the same CRSS (one load)
+        const double t = 1.0 / crss[v];
+#pragma unroll
+        for (int k = 0; k < kSlips; k++) it[k] = t;
+      } else {
+        load_inv_tau(crss, n, v, it);
+      }
+      double s[5];
+#pragma unroll
+      for (int i = 0; i < 5; i++) s[i] = sigma[i * n + v];
+      // yield-surface rescale, MaterialGrid.cpp:954-959
+      double mrss, sc[5];
+      {
+        double Q[25];
+        cf::dev_map(r9, Q);
+        cf::rot(Q, s, sc);
+        mrss = cf::max_rss(sc, it);
+      }
+      if (!(mrss >= 1e-10)) acc[13] += 1.0;
+      const double imrss = rcp_nr(mrss);
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        s[i] /= mrss;
+        sc[i] *= imrss;  // the crystal-frame start point of the solve
+      }
+      // q = L0^-1 lambda + edot + d~ ; lambda and d~ are re-read after the solve
+      double qv[5];
+#pragma unroll
+      for (int i = 0; i < 5; i++) qv[i] = C.edot[i] + (FUSED ? (double)fld[i * 256 + slot] : ld(dtil + i * n + v));
+#pragma unroll
+      for (int j = 0; j < 5; j++) {
+        const double lj = lambda[j * n + v];
+#pragma unroll
+        for (int i = 0; i < 5; i++) qv[i] += C.L0inv[j * 5 + i] * lj;
+      }
+      double err = 0.0;
+      int steps = 0;
+      {
+        double P[25], qc[5], A0[15];
+        cf::inverse_map(r9, P);
+        cf::rot_t(P, qv, qc);
+        cf::congruence_t(P, C.L0inv, A0);
+        cf::constrained_nr<NE>(s, sc, r9, P, it, C, qc, A0, 5e-4, 1e9, 100, &err, &steps);
+      }
+      double e[5];
+      {
+        double Q[25], sc[5], ec[5];
+        cf::dev_map(r9, Q);
+        cf::rot(Q, s, sc);
+        cf::apply_constitutive<NE>(sc, it, C, ec);
+        cf::rot_t(Q, ec, e);
+      }
+      // re-read d~ and lambda here rather than keep them live through the solve
+      __asm__ volatile("" ::: "memory");
+      double d[5], lam[5];
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        d[i] = FUSED ? (double)fld[i * 256 + slot] : ld(dtil + i * n + v);
+        lam[i] = lambda[i * n + v];
+      }
+      double Ld[5], t[5], Lt[5];
+      matvec5(C.L0, d, Ld);
+#pragma unroll
+      for (int i = 0; i < 5; i++) t[i] = C.edot[i] + d[i] - e[i];
+      matvec5(C.L0, t, Lt);
+      double se2 = 0.0, ee2 = 0.0;
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        lam[i] += Lt[i];
+        if constexpr (FUSED) fld[i * 256 + slot] = (T)(s[i] - Ld[i]);
+        else phi[i * n + v] = (T)(s[i] - Ld[i]);
+        sigma[i * n + v] = s[i];
+        lambda[i * n + v] = lam[i];
+        const double se = s[i] - lam[i];
+        const double ee = e[i] - C.edot[i] - d[i];
+        se2 += se * se;
+        ee2 += ee * ee;
+        acc[i] += s[i];
+        acc[5 + i] += e[i];
+      }
+      acc[10] += sqrt(se2);
+      acc[11] += sqrt(ee2);
+      acc[12] += (double)steps;
+      mx[0] = fmax(mx[0], err);
+      mx[1] = fmax(mx[1], (double)steps);
+      // the first non-finite value of the step: the FFT's output, then what
+      // this kernel made of it
+      const double fd = sum5(d), fs = sum5(s), fe = sum5(e), fl = sum5(lam);
+      if (__builtin_expect(!__builtin_isfinite(fd + fs + fe + fl), 0)) {
+        const int f = !__builtin_isfinite(fd) ? HF_DTILDE
+                      : !__builtin_isfinite(fs) ? HF_STRESS
+                      : !__builtin_isfinite(fe) ? HF_STRAIN_RATE : HF_LAMBDA;
+        const double val = f == HF_DTILDE ? fd : f == HF_STRESS ? fs : f == HF_STRAIN_RATE ? fe : fl;
+        report_health(health, HK_NR, f, ls ? ls->iter + 1 : 0, v, val);
+      }
+    }
+    // lane efficiency of the Newton loop: a wave runs as many steps as its
+    // slowest lane, the wave's maximum of the steps (slot 14 is 0 in every
+    // lane before the reduction and 64 x that maximum after it)
+    wave_sum<15>(acc, mx, partial + blk * kPartialStride);
+    if ((threadIdx.x & 63) == 63) partial[blk * kPartialStride + 14] = 64.0 * mx[1];
+  }
+  if constexpr (FUSED) {  // phi row: forward z transforms from LDS
+    __syncthreads();
+    // the pencil coordinates afresh: the compiler would otherwise keep the
+    // inverse transforms' twiddle addresses (the same tw[t k]) live through
+    // the body, in scratch
+    int t3 = threadIdx.x & 15;
+    __asm__ volatile("" : "+v"(t3));
+    Cx v[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const T a = zactive ? fld[(2 * zpair) * 256 + t3 + 16 * j] : (T)0;
+      const T b = (zactive && zpair < 2) ? fld[(2 * zpair + 1) * 256 + t3 + 16 * j] : (T)0;
+      v[j] = cmake<T>(a, b);
+    }
+    Cx* so = (Cx*)zf.spec_out;
+    Cx* sa = so + (long)(2 * zpair) * zf.nc + zrow * zf.nzp;
+    Cx* sb = zpair < 2 ? so + (long)(2 * zpair + 1) * zf.nc + zrow * zf.nzp : nullptr;
+    z_r2c256_pencil<T, 3>(v, zactive, (const Cx*)zf.tw, zbuf, zpair, t3, sa, sb);
+  }
 }
 
+// phi = sigma - L0 d~ after the loop of a fused step (the fused kernel keeps
+// both rows in LDS): the values its body wrote there, by the same operations.
+template <typename T>
+__global__ void __launch_bounds__(256) k_phi_rebuild(long n, const T* __restrict__ dtil, T* __restrict__ phi,
+                                                     const double* __restrict__ sigma, const StepConst* __restrict__ Cp) {
+  const StepConst& C = *as_const_mem(Cp);
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
+    double d[5], Ld[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) d[i] = ld(dtil + i * n + v);
+    matvec5(C.L0, d, Ld);
+#pragma unroll
+    for (int i = 0; i < 5; i++) phi[i * n + v] = (T)(sigma[i * n + v] - Ld[i]);
+  }
+}
 
 // Reduce rows [256 b, 256 b + 256) of the per-block partials into row b of
 // out: 15 sums then 2 maxima, fixed tree order.
@@ -1115,11 +1212,13 @@ hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenCon
 
 hipError_t launch_nr(int prec, int nexp_uniform, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
                      double* lambda, const double* R, const double* crss, const StepConst* C, double* partial,
-                     const LoopState* ls, HealthRecord* health, hipStream_t st) {
+                     const LoopState* ls, HealthRecord* health, bool linear, hipStream_t st) {
   const int nb = nr_blocks(n);
+  ZFusedArgs zf{};
+  zf.linear = linear ? 1 : 0;
 #define NR_LAUNCH(TT, NN, UU)                                                                                    \
-  hipLaunchKernelGGL((k_nr<TT, NN, UU, false>), dim3(nb), dim3(kNrThreads), 0, st, n, (const TT*)dtil, (TT*)phi, \
-                     sigma, lambda, R, crss, C, partial, ls, health, VoxGroup{})
+  hipLaunchKernelGGL((k_nr<TT, NN, UU, NR_SINGLE>), dim3(nb), dim3(kNrThreads), 0, st, n, (const TT*)dtil,      \
+                     (TT*)phi, sigma, lambda, R, crss, C, partial, ls, health, VoxGroup{}, zf)
 #define NR_N(TT, UU)                                 \
   switch (nexp_uniform) {                             \
     case 5: NR_LAUNCH(TT, 5, UU); break;              \
@@ -1140,6 +1239,53 @@ hipError_t launch_nr(int prec, int nexp_uniform, bool uniform_tau, long n, const
   const int nb2 = (nb + 255) / 256;
   hipLaunchKernelGGL(k_fold_partials, dim3(nb2), dim3(256), 0, st, partial, nb, partial + (long)nb * kPartialStride,
                      ls ? &ls->done : nullptr);
+  return hipGetLastError();
+}
+
+bool zfused_eligible(const FftGeom& g) { return g.nz == 256; }
+
+hipError_t launch_zfused(int prec, int nexp_uniform, bool uniform_tau, long n, const void* spec_in, void* spec_out,
+                         const FftGeom& g, const void* tw, double scale, double* sigma, double* lambda,
+                         const double* R, const double* crss, const StepConst* C, double* partial,
+                         const LoopState* ls, HealthRecord* health, hipStream_t st) {
+  if (!zfused_eligible(g)) return hipErrorInvalidValue;
+  const unsigned blocks = (unsigned)g.nrows;
+  const ZFusedArgs zf{spec_in, spec_out, tw, g.nc, g.nzp, scale, 0};
+#define ZF_LAUNCH(TT, NN, UU)                                                                                        \
+  hipLaunchKernelGGL((k_nr<TT, NN, UU, NR_ZFUSED>), dim3(blocks), dim3(kNrThreads), 0, st, n, (const TT*)nullptr,    \
+                     (TT*)nullptr, sigma, lambda, R, crss, C, partial, ls, health, VoxGroup{}, zf)
+#define ZF_N(TT, UU)                                 \
+  switch (nexp_uniform) {                             \
+    case 5: ZF_LAUNCH(TT, 5, UU); break;              \
+    case 10: ZF_LAUNCH(TT, 10, UU); break;            \
+    case 20: ZF_LAUNCH(TT, 20, UU); break;            \
+    default: ZF_LAUNCH(TT, 0, UU); break;             \
+  }
+  if (prec == 0) {
+    if (uniform_tau) { ZF_N(double, true) } else { ZF_N(double, false) }
+  } else {
+    if (uniform_tau) { ZF_N(float, true) } else { ZF_N(float, false) }
+  }
+#undef ZF_N
+#undef ZF_LAUNCH
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // the same partial rows as k_nr's, folded the same way
+  const int nb = nr_blocks(n);
+  const int nb2 = (nb + 255) / 256;
+  hipLaunchKernelGGL(k_fold_partials, dim3(nb2), dim3(256), 0, st, partial, nb, partial + (long)nb * kPartialStride,
+                     ls ? &ls->done : nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_phi_rebuild(int prec, long n, const void* dtil, void* phi, const double* sigma, const StepConst* C,
+                              hipStream_t st) {
+  if (prec == 0)
+    hipLaunchKernelGGL(k_phi_rebuild<double>, dim3(point_blocks(n)), dim3(256), 0, st, n, (const double*)dtil,
+                       (double*)phi, sigma, C);
+  else
+    hipLaunchKernelGGL(k_phi_rebuild<float>, dim3(point_blocks(n)), dim3(256), 0, st, n, (const float*)dtil,
+                       (float*)phi, sigma, C);
   return hipGetLastError();
 }
 
@@ -1226,9 +1372,9 @@ hipError_t launch_loop_reset_group(const VoxGroup& grp, hipStream_t st) {
 hipError_t launch_nr_group(int prec, int nexp_uniform, bool uniform_tau, const VoxGroup& grp, long blocks,
                            hipStream_t st) {
 #define NR_LAUNCH(TT, NN, UU)                                                                                   \
-  hipLaunchKernelGGL((k_nr<TT, NN, UU, true>), dim3((unsigned)blocks), dim3(kNrThreads), 0, st, 0L,             \
+  hipLaunchKernelGGL((k_nr<TT, NN, UU, NR_BATCHED>), dim3((unsigned)blocks), dim3(kNrThreads), 0, st, 0L,       \
                      (const TT*)nullptr, (TT*)nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, \
-                     nullptr, grp)
+                     nullptr, grp, ZFusedArgs{})
 #define NR_N(TT, UU)                                 \
   switch (nexp_uniform) {                             \
     case 5: NR_LAUNCH(TT, 5, UU); break;              \
