@@ -1,9 +1,13 @@
 // facade_check.cpp -- drives VPFFT::DataStructures::MaterialGrid
 // (include/vpfft_materialgrid.hpp) the way the reference's MPI_VPFFT_Test
-// drives its own class (Src/UnitTester.cpp:384-457): the reference's
-// material, a seeded polycrystal, tension, consecutive strain steps.  Prints
-// one line per result with 17 significant digits for the Python test to
-// compare with the same run through the Python mirror.
+// drives its own class (Src/UnitTester.cpp:384-457), with the same calls on
+// the grid and on the value types: FCC_SchmidtBasis::Get, the three-integer
+// constructor (the reference's material), SendDataAndInitialize,
+// SetVoxelLength, a strain rate built with SMatrix3x3::SetZero and
+// EigenRep(const SMatrix3x3&), consecutive strain steps.  The orientations
+// are a seeded polycrystal.  Prints one line per result with 17 significant
+// digits for the Python test to compare with the same run through the Python
+// mirror.
 //
 // usage: facade_check <edge> <grains> <seed> <steps>
 #include <cmath>
@@ -16,6 +20,7 @@
 
 using VPFFT::Error;
 using VPFFT::DataStructures::MaterialGrid;
+using VPFFT::FCC_CrystalTest::FCC_SchmidtBasis;
 using VPFFT::LinearAlgebra::EigenRep;
 using VPFFT::LinearAlgebra::SMatrix3x3;
 
@@ -39,11 +44,19 @@ int main(int argc, char** argv) {
   std::vector<SMatrix3x3> orient(n);
   std::memcpy(orient.data(), R.data(), sizeof(double) * 9 * n);
   try {
+    const FCC_SchmidtBasis& fcc = FCC_SchmidtBasis::Get();
+    std::printf("slip_systems %d %.17g\n", FCC_SchmidtBasis::NumSlipSystems(), fcc(0).m[0][1]);
     MaterialGrid grid(e, e, e);  // the reference's hard-coded material
-    grid.SetVoxelLength(1, 1, 1);
     grid.SendDataAndInitialize(orient.data(), nullptr);
-    // proj5(diag(1, -1/2, -1/2)): uniaxial tension along x
-    const EigenRep edot(-1.0606601717798212, -0.61237243569579458, 0, 0, 0);
+    grid.SetVoxelLength(1, 1, 1);
+    // uniaxial tension along x
+    SMatrix3x3 StrainRateM;
+    StrainRateM.SetZero();
+    StrainRateM.m[0][0] = 1;
+    StrainRateM.m[1][1] = -0.5;
+    StrainRateM.m[2][2] = -0.5;
+    const EigenRep edot(StrainRateM);
+    print5("edot", edot.m);
     for (int s = 0; s < steps; s++) {
       const EigenRep avg = grid.RunSingleStrainStep(edot, 100, 0.005, 1e-5);
       std::printf("iterations %d\n", grid.LastStepInfo().iterations);

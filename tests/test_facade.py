@@ -51,7 +51,7 @@ def test_facade_driver_matches_python_mirror(gpu, vpfft):
         g.SetVoxelLength(1.0, 1.0, 1.0)
         g.SendDataAndInitialize(R)
         for s in range(steps):
-            avg = g.RunSingleStrainStep(edot_tension(), 100, 0.005, 1e-5)
+            avg = g.RunSingleStrainStep(r["edot"], 100, 0.005, 1e-5)
             assert r["iterations"][s] == g.last_info.iterations
             assert np.array_equal(r["avg"][s], avg), s
         v = g.ToIndex(1, 2, 3) if hasattr(g, "ToIndex") else (1 * edge + 2) * edge + 3
@@ -61,4 +61,7 @@ def test_facade_driver_matches_python_mirror(gpu, vpfft):
         o = g.Orientation()[v]
         assert np.array_equal(r["orientation"], o[[0, 4, 8]])
     assert r["roundtrip"][0] == 1
+    # the strain rate built by EigenRep(const SMatrix3x3&) is the tension one
+    assert np.max(np.abs(r["edot"] - edot_tension())) < 1e-15
+    assert r["slip_systems"][0] == 12 and abs(r["slip_systems"][1] - 1 / (2 * 6 ** 0.5)) < 1e-16
     assert r["degenerate"][0] == -4  # VPFFT_E_DEGENERATE
