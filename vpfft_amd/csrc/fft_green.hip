@@ -927,9 +927,10 @@ __device__ __forceinline__ void fft256_x(typename C2<T>::t v[16], T* buf, const 
   dft16<T>(v, sign);
 }
 
-// Four waves per SIMD: three blocks per CU.  In FP64 that spills 52 B per
-// lane and still runs 14% faster than three waves (two blocks per CU)
-// without spills (256^3: 0.53 against 0.61 ms).
+// Four waves per SIMD: three blocks per CU.  In FP64 that spills 44 B per
+// lane (FFT values of the second half through the first half's Green
+// phase) and still runs faster than three waves (two blocks per CU) without
+// spills (256^3: 0.53 against 0.61 ms).
 template <typename T, int P>
 __global__ void __launch_bounds__(80 * P) __attribute__((amdgpu_waves_per_eu(4))) k_x256_green(typename C2<T>::t* __restrict__ s, FftGeom g,
                                                         const typename C2<T>::t* __restrict__ tw,
@@ -1004,9 +1005,14 @@ __global__ void __launch_bounds__(80 * P) __attribute__((amdgpu_waves_per_eu(4))
   const C* tw2 = tw;
   __asm__ volatile("" : "+s"(tw2));
   fft256_x<T, P>(v, cbuf, tw2, t, p, +1);
-  if (active) {
+  // the thread's pencil coordinates afresh for the stores, rather than kept
+  // live (in scratch) through the transforms and the Green phase
+  int r2 = threadIdx.x - c * 16 * P;
+  __asm__ volatile("" : "+v"(r2));
+  const int p2 = r2 % P, t2 = r2 / P;
+  if (kz0 + p2 < g.nzh) {
 #pragma unroll
-    for (int k2 = 0; k2 < 16; k2++) base[(unsigned)(t + 16 * k2) * ncol + (unsigned)p] = v[k2];
+    for (int k2 = 0; k2 < 16; k2++) base[(unsigned)(t2 + 16 * k2) * ncol + (unsigned)p2] = v[k2];
   }
 }
 
