@@ -139,6 +139,10 @@ struct Grid {
   // hardening law whose matrix rows and Voce constants are slip-independent)
   bool crss_uniform = false;
   bool hardening_uniform = false;
+  bool gdot0_uniform = false;  // every slip system has the same reference slip rate
+  // the Newton kernel's uniform variant: one CRSS and one reference slip rate
+  // for all slip systems of a voxel
+  bool uniform_slips() const { return crss_uniform && gdot0_uniform; }
   // fixed-point loop: device loop state and a captured graph of several
   // iterations (VPFFT_GRAPH_ITERS, 0 = launch every kernel from the host)
   LoopState* dls = nullptr;
@@ -436,6 +440,9 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
   g->prec = d->precision;
   g->mat = *mat;
   if (const char* k = std::getenv("VPFFT_GRAPH_ITERS")) g->graph_iters = std::max(0, std::min(64, std::atoi(k)));
+  g->gdot0_uniform = true;
+  for (int k = 0; k < kSlips; k++)
+    if (mat->gamma_dot_base[k] != mat->gamma_dot_base[0]) g->gdot0_uniform = false;
   g->hardening_uniform = true;
   for (int i = 0; i < kSlips; i++) {
     if (mat->tau0[i] != mat->tau0[0] || mat->tau1[i] != mat->tau1[0] || mat->theta0[i] != mat->theta0[0] ||
@@ -565,13 +572,13 @@ static int enqueue_iteration(Grid* g, int prec, double tol, int max_iter, int k)
     HIP_TRY(launch_y(prec, sin, g->geom, g->plan[1], tw[1], +1, g->fftB[prec][1], done, g->st));
     kend(g, K_Y_INV);
     kbegin(g, K_ZNR);
-    HIP_TRY(launch_zfused(prec, g->nexp_uniform, g->crss_uniform, g->n, sin, sout, g->geom, tw[2], 1.0 / (double)g->n,
+    HIP_TRY(launch_zfused(prec, g->nexp_uniform, g->uniform_slips(), g->n, sin, sout, g->geom, tw[2], 1.0 / (double)g->n,
                           g->sigma, g->lambda, g->R, g->crss, g->dC, g->partial, g->dls, g->dhealth, g->st));
     kend(g, K_ZNR);
   } else {
     if ((rc = run_fft_green(g, prec, done))) return rc;
     kbegin(g, K_NR);
-    HIP_TRY(launch_nr(prec, g->nexp_uniform, g->crss_uniform, g->n, g->dtil, g->phi, g->sigma, g->lambda, g->R,
+    HIP_TRY(launch_nr(prec, g->nexp_uniform, g->uniform_slips(), g->n, g->dtil, g->phi, g->sigma, g->lambda, g->R,
                       g->crss, g->dC, g->partial, g->dls, g->dhealth, g->nz == 256, g->st));
     kend(g, K_NR);
   }
@@ -674,7 +681,7 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
   // even number of iterations
   const int graph_iters = fused ? (g->graph_iters + 1) / 2 * 2 : g->graph_iters;
   if (use_graph) {
-    const Grid::GraphKey key{prec, max_iter, g->nexp_uniform, (g->crss_uniform ? 1 : 0) | (fused ? 2 : 0), tol};
+    const Grid::GraphKey key{prec, max_iter, g->nexp_uniform, (g->uniform_slips() ? 1 : 0) | (fused ? 2 : 0), tol};
     if (!g->gexec || !(g->gkey == key)) {
       if (g->gexec) {
         HIP_TRY(hipGraphExecDestroy(g->gexec));
@@ -1014,7 +1021,7 @@ static int batch_plan(Batch* b, Grid** gs, int count, const double* dt, const st
   std::vector<int> epi;
   for (int j = 0; j < count; j++) {
     if (!active[j]) continue;
-    nr[std::make_tuple(gs[j]->prec, gs[j]->nexp_uniform, gs[j]->crss_uniform ? 1 : 0)].push_back(j);
+    nr[std::make_tuple(gs[j]->prec, gs[j]->nexp_uniform, gs[j]->uniform_slips() ? 1 : 0)].push_back(j);
     if (dt[j] > 0) epi.push_back(j);
   }
   // FFT passes: groups by (precision, block shape)
@@ -1216,7 +1223,7 @@ static int batch_step(Batch* b, Grid** gs, int count, const double* edot, int ma
   }
   for (int j = 0; j < count; j++) {
     key_vals.push_back(gs[j]->prec);
-    key_vals.push_back(gs[j]->crss_uniform);
+    key_vals.push_back(gs[j]->uniform_slips());
   }
   if (b->graph_iters > 0) {
     if (!b->gexec || key_grids != b->gkey_grids || key_vals != b->gkey_vals || tol != b->gkey_tol) {

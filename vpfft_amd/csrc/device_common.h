@@ -183,15 +183,17 @@ __device__ __forceinline__ double rcp_nr(double d) {
 }
 
 // In-place variant: A's packed lower triangle is overwritten by L (unit
-// diagonal implied), the solution replaces b.  The arithmetic of ldlt_solve5,
-// with the pivots' reciprocals from rcp_nr.
+// diagonal implied), the solution replaces b.  The factorisation of
+// ldlt_solve5 with the same pivot rule, but each column is formed from the
+// unscaled entries U = L D of the rows above it (one FMA per term instead of
+// two products), and the pivots' reciprocals come from rcp_nr.
 __device__ __forceinline__ void ldlt_solve5_inplace(double A[15], double b[5]) {
-  double dinv[5];
+  double dinv[5], U[15];
 #pragma unroll
   for (int j = 0; j < 5; j++) {
     double d = A[sidx(j, j)];
 #pragma unroll
-    for (int k = 0; k < j; k++) d -= A[sidx(j, k)] * A[sidx(j, k)] * A[sidx(k, k)];
+    for (int k = 0; k < j; k++) d = __builtin_fma(-A[sidx(j, k)], U[sidx(j, k)], d);
     const double scale = fabs(A[sidx(j, j)]);
     const bool ok = d > 1e-300 && d > 1e-14 * scale;
     const double dj = ok ? d : 0.0;
@@ -200,7 +202,8 @@ __device__ __forceinline__ void ldlt_solve5_inplace(double A[15], double b[5]) {
     for (int i = j + 1; i < 5; i++) {
       double t = A[sidx(i, j)];
 #pragma unroll
-      for (int k = 0; k < j; k++) t -= A[sidx(i, k)] * A[sidx(j, k)] * A[sidx(k, k)];
+      for (int k = 0; k < j; k++) t = __builtin_fma(-A[sidx(i, k)], U[sidx(j, k)], t);
+      U[sidx(i, j)] = t;
       A[sidx(i, j)] = t * dinv[j];
     }
     A[sidx(j, j)] = dj;  // D stored on the diagonal
@@ -515,59 +518,113 @@ __device__ __forceinline__ void schmid_sums(const double g[kSlips], const double
   S[4] = Ga[0] + Gb[2];
 }
 
+// U (uniform slip systems): every slip system of the voxel has the same CRSS
+// and the same reference slip rate, inv_tau[0] and C.gdot0[0].  Then 1/tau is
+// folded into the scaled stress (5 products instead of 12) and gdot0 and the
+// exponent into the sums' constants.
+
+// u = w * x, times 1/tau when U
+template <bool U>
+__device__ __forceinline__ void scaled_u(const double x[5], const double inv_tau[kSlips], double u[5]) {
+#pragma unroll
+  for (int i = 0; i < 5; i++) u[i] = U ? (w(i) * inv_tau[0]) * x[i] : w(i) * x[i];
+}
+
 // max_k |m0_k . sc| / tau_k (MaterialGrid.h:550-566)
+template <bool U = false>
 __device__ __forceinline__ double max_rss(const double sc[5], const double inv_tau[kSlips]) {
   double u[5];
-  scaled(sc, u);
+  scaled_u<U>(sc, inv_tau, u);
   double mx = 0.0;
 #pragma unroll
-  for (int k = 0; k < kSlips; k++) mx = fmax(mx, fabs(schmid_dot(k, u) * inv_tau[k]));
+  for (int k = 0; k < kSlips; k++) mx = fmax(mx, fabs(U ? schmid_dot(k, u) : schmid_dot(k, u) * inv_tau[k]));
   return mx;
 }
 
+// G = P^T P (packed lower): |P x|^2 = x^T G x for crystal-frame x.
+__device__ __forceinline__ void gram(const double P[25], double G[15]) {
+#pragma unroll
+  for (int a = 0; a < 5; a++)
+#pragma unroll
+    for (int c = 0; c <= a; c++) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < 5; i++) t += P[i * 5 + a] * P[i * 5 + c];
+      G[sidx(a, c)] = t;
+    }
+}
+
+// y = G x for a symmetric G (packed lower)
+__device__ __forceinline__ void symv(const double G[15], const double x[5], double y[5]) {
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < 5; j++) t += G[sidx(i, j)] * x[j];
+    y[i] = t;
+  }
+}
+
+// x^T G x for a symmetric G (packed lower)
+__device__ __forceinline__ double quad(const double G[15], const double x[5]) {
+  double acc = 0.0;
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < i; j++) t += G[sidx(i, j)] * x[j];
+    acc += x[i] * __builtin_fma(2.0, t, G[sidx(i, i)] * x[i]);
+  }
+  return acc;
+}
+
 // Solvers::SolveConstrainedConstitutiveEquations (Solvers.cpp:111-222) with
-// the Newton matrix built in the crystal frame.  cur: the sample-frame
-// iterate (in/out); cc: the same point in the crystal frame, Q cur (in;
-// overwritten); R: the orientation (Q = Q(R) is rebuilt on a back-track);
-// P = Q^-1; q = P^T (L0^-1 lambda + edot + d~); A0 = P^T L0^-1 P (packed
-// lower).  Budget, back-track and stopping rule as in the reference;
-// *err_out untouched when the budget runs out in the back-track.
-template <int NE>
-__device__ __forceinline__ void constrained_nr(double cur[5], double cc[5], const double R[9], const double P[25],
-                                               const double inv_tau[kSlips], const StepConst& C, const double q[5],
-                                               const double A0[15], double eps, double max_rss, int max_it,
-                                               double* err_out, int* steps_out) {
+// the Newton matrix built, and the iterate kept, in the crystal frame of the
+// voxel.  cc: the start point Q sigma (in), the solution (out; the caller maps
+// it back, sigma = P cc = Q^T G cc); G = P^T P with P = Q^-1, so the sample-
+// frame step and midpoint norms of the stopping rule are x^T G x; q = P^T
+// (L0^-1 lambda + edot + d~); A0 = P^T L0^-1 P (packed lower).  A back-track
+// moves cc towards the previous iterate: the map is linear, so that is the
+// reference's sample-frame back-track.  Budget, back-track factor and stopping
+// rule as in the reference; *err_out untouched when the budget runs out in
+// the back-track.  Neither the orientation, nor P, nor a sample-frame iterate
+// is live in the loop.
+template <int NE, bool U = false>
+__device__ __forceinline__ void constrained_nr(double cc[5], const double G[15], const double inv_tau[kSlips],
+                                               const StepConst& C, const double q[5], const double A0[15],
+                                               double eps, double max_rss, int max_it, double* err_out,
+                                               int* steps_out) {
   double saved[5];
 #pragma unroll
-  for (int i = 0; i < 5; i++) saved[i] = cur[i];
+  for (int i = 0; i < 5; i++) saved[i] = cc[i];
   int rem = max_it;
   int steps = 0;
   double dn = 1.0, an = 1e-10;  // rel = 1e5 until a step is taken (Solvers.cpp:137)
   const double eps2 = eps * eps;
+  // U: S and J below are formed from |r|^(n-1) alone; the residual takes
+  // them times gdot0, the matrix times n gdot0 / tau
+  const double cs = U ? C.gdot0[0] : 1.0, cj = U ? (double)NE * C.gdot0[0] * inv_tau[0] : 1.0;
   while (rem > 0) {
     double r[kSlips];
     bool ok;
     do {
       double u[5];
-      scaled(cc, u);
+      scaled_u<U>(cc, inv_tau, u);
       double mx = 0.0;
 #pragma unroll
       for (int k = 0; k < kSlips; k++) {
-        r[k] = schmid_dot(k, u) * inv_tau[k];
+        r[k] = U ? schmid_dot(k, u) : schmid_dot(k, u) * inv_tau[k];
         mx = fmax(mx, fabs(r[k]));
       }
       ok = !(mx > max_rss);
       if (!ok) {
 #pragma unroll
-        for (int i = 0; i < 5; i++) cur[i] = saved[i] + (cur[i] - saved[i]) * 0.2;
-        double Q[25];
-        dev_map(R, Q);
-        rot(Q, cur, cc);
+        for (int i = 0; i < 5; i++) cc[i] = saved[i] + (cc[i] - saved[i]) * 0.2;
       }
       rem--;
       if (rem < 0) {
         *steps_out = steps;
-        return;  // cur is the returned state (Solvers.cpp:169)
+        return;  // cc is the returned state (Solvers.cpp:169)
       }
     } while (!ok);
     steps++;
@@ -575,40 +632,42 @@ __device__ __forceinline__ void constrained_nr(double cur[5], double cc[5], cons
     double gr[kSlips], ck[kSlips], S[5], J[15];
 #pragma unroll
     for (int k = 0; k < kSlips; k++) {
-      const double g = gamma_nm1<NE>(C, k, fabs(r[k]));
-      gr[k] = g * r[k];
-      ck[k] = nexp<NE>(C, k) * g * inv_tau[k];
+      if constexpr (U) {
+        const double p = NE != 1 ? ipow_c<NE - 1>(fabs(r[k])) : 1.0;
+        gr[k] = p * r[k];
+        ck[k] = p;
+      } else {
+        const double g = gamma_nm1<NE>(C, k, fabs(r[k]));
+        gr[k] = g * r[k];
+        ck[k] = nexp<NE>(C, k) * g * inv_tau[k];
+      }
     }
     schmid_sums(gr, ck, S, J);
-    // rhs_c = q - A0 cc - S_c ;  (A0 + J_c) d_c = rhs_c ; d = P d_c
+    // rhs_c = q - A0 cc - S_c ;  (A0 + J_c) d_c = rhs_c
     double rhs[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) {
       double Ac = 0.0;
 #pragma unroll
       for (int j = 0; j < 5; j++) Ac += A0[i >= j ? sidx(i, j) : sidx(j, i)] * cc[j];
-      rhs[i] = q[i] - Ac - w(i) * S[i];
+      rhs[i] = q[i] - Ac - (U ? w(i) * cs : w(i)) * S[i];
     }
 #pragma unroll
     for (int i = 0; i < 5; i++)
 #pragma unroll
-      for (int j = 0; j <= i; j++) J[sidx(i, j)] = A0[sidx(i, j)] + (w(i) * w(j)) * J[sidx(i, j)];
+      for (int j = 0; j <= i; j++)
+        J[sidx(i, j)] = A0[sidx(i, j)] + (U ? (w(i) * w(j)) * cj : w(i) * w(j)) * J[sidx(i, j)];
     ldlt_solve5_inplace(J, rhs);
-    double ds[5];
-    rot(P, rhs, ds);
-    dn = 0.0;
-    an = 0.0;
+    // the sample-frame step P d_c and midpoint P (cc + d_c / 2), by their norms
+    double mid[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) {
-      saved[i] = cur[i];
-      const double nw = cur[i] + ds[i];
-      const double diff = nw - saved[i];
-      const double avg = (nw + saved[i]) * 0.5;
-      dn += diff * diff;
-      an += avg * avg;
-      cur[i] = nw;
+      saved[i] = cc[i];
+      mid[i] = __builtin_fma(0.5, rhs[i], cc[i]);
       cc[i] += rhs[i];
     }
+    dn = quad(G, rhs);
+    an = quad(G, mid);
     // rel = |d| / |avg| < eps, compared squared; rel itself is formed once at exit
     if (dn < eps2 * an) break;
   }
@@ -617,17 +676,17 @@ __device__ __forceinline__ void constrained_nr(double cur[5], double cc[5], cons
 }
 
 // Solvers::ApplyConstitutiveEquations (Solvers.cpp:255-274), crystal frame.
-template <int NE>
+template <int NE, bool U = false>
 __device__ __forceinline__ void apply_constitutive(const double sc[5], const double inv_tau[kSlips],
                                                    const StepConst& C, double ec[5]) {
   double u[5], a[5];
-  scaled(sc, u);
+  scaled_u<U>(sc, inv_tau, u);
 #pragma unroll
   for (int i = 0; i < 5; i++) a[i] = -0.0;
 #pragma unroll
   for (int k = 0; k < kSlips; k++) {
-    const double r = schmid_dot(k, u) * inv_tau[k];
-    const double g = gamma_n<NE>(C, k, fabs(r)) * sgn(r);
+    const double r = U ? schmid_dot(k, u) : schmid_dot(k, u) * inv_tau[k];
+    const double g = U ? ipow_c<NE>(fabs(r)) * sgn(r) : gamma_n<NE>(C, k, fabs(r)) * sgn(r);
 #pragma unroll
     for (int i = 0; i < 5; i++) {
       const int si = kSign[k][i];
@@ -638,7 +697,7 @@ __device__ __forceinline__ void apply_constitutive(const double sc[5], const dou
     }
   }
 #pragma unroll
-  for (int i = 0; i < 5; i++) ec[i] = w(i) * a[i];
+  for (int i = 0; i < 5; i++) ec[i] = (U ? w(i) * C.gdot0[0] : w(i)) * a[i];
 }
 
 }  // namespace cf

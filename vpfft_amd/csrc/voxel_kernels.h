@@ -94,6 +94,7 @@ hipError_t launch_health_reset(HealthRecord* health, hipStream_t st);
 hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenConst<float>* g32, hipStream_t st);
 // waves: 1 or 2 waves per SIMD (register budget 512 or 256 per lane);
 // uniform_tau: every slip system of a voxel has the same CRSS (crss[0][v] is read)
+// and the same reference slip rate (C.gdot0[0] is read)
 // ls: the loop state (kernels return at entry once ls->done is set; may be
 // null outside a loop); health: where the first non-finite value is recorded;
 // linear: each wave takes 64 consecutive voxels (the fused kernel's runs)

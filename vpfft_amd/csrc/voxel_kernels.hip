@@ -112,36 +112,81 @@ __device__ __forceinline__ double dpp64(double x) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// Sum (or maximum) over one full wave in a fixed order, by DPP moves: within
-// quads, within rows of 16 (rotations by 4 and 8), then rows 1 and 3 take
-// lane 15 / 47 and rows 2 and 3 lane 31.  Lane 63 holds the result.
-template <bool MAX>
-__device__ __forceinline__ double wave_reduce(double v) {
-  auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
-  v = op(v, dpp64<0xB1>(v));        // quad_perm [1,0,3,2]
-  v = op(v, dpp64<0x4E>(v));        // quad_perm [2,3,0,1]
-  v = op(v, dpp64<0x124>(v));       // row_ror:4
-  v = op(v, dpp64<0x128>(v));       // row_ror:8
-  v = op(v, dpp64<0x142, 0xa>(v));  // row_bcast:15
-  v = op(v, dpp64<0x143, 0xc>(v));  // row_bcast:31
-  return v;
+// One step of a pairwise exchange inside rows of 16 lanes: the lane and its
+// partner (the DPP pattern CTRL pairs them, and HIGH tells them apart) each
+// keep one half of the H pairs (lo[j], hi[j]) and add the partner's copy of
+// that half.  Both lanes of a pair add the same two numbers.
+template <int CTRL, int H>
+__device__ __forceinline__ void pair_halve(const double (&lo)[H], const double (&hi)[H], bool high,
+                                           double (&out)[H]) {
+#pragma unroll
+  for (int j = 0; j < H; j++) {
+    const double send = high ? lo[j] : hi[j];
+    const double keep = high ? hi[j] : lo[j];
+    out[j] = keep + dpp64<CTRL>(send);
+  }
 }
 
-// Reduce W sums and M maxima over one wave in a fixed order; lane 63 writes
-// them to out[0..W+M).  The maxima must be of values >= 0: lanes of rows a
-// broadcast leaves out read 0.
-template <int W, int M>
-__device__ __forceinline__ void wave_sum(double (&v)[W], double (&vmax)[M], double* out) {
+// x of this lane's row plus x of the paired row (rows 0-1, 2-3), then of the
+// other half of the wave, by lane swaps; every lane gets the total over its
+// lane index mod 16.  ADD selects + (else max).
+template <bool ADD>
+__device__ __forceinline__ double cross_rows(double x) {
+  auto op = [](double a, double b) { return ADD ? a + b : fmax(a, b); };
+  long long b = __double_as_longlong(x);
+  auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+  auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  x = op(__longlong_as_double(((long long)hi[0] << 32) | lo[0]),
+         __longlong_as_double(((long long)hi[1] << 32) | lo[1]));
+  b = __double_as_longlong(x);
+  lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+  hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+  return op(__longlong_as_double(((long long)hi[0] << 32) | lo[0]),
+            __longlong_as_double(((long long)hi[1] << 32) | lo[1]));
+}
+
+// Maximum over one wave; every lane gets it.
+__device__ __forceinline__ double wave_max_all(double x) {
+  x = fmax(x, dpp64<0xB1>(x));   // quad_perm [1,0,3,2]
+  x = fmax(x, dpp64<0x1B>(x));   // quad_perm [3,2,1,0]
+  x = fmax(x, dpp64<0x141>(x));  // row_half_mirror
+  x = fmax(x, dpp64<0x140>(x));  // row_mirror
+  return cross_rows<false>(x);
+}
+
+// The per-wave row of the Newton kernel's partial sums: 14 sums of acc (slot
+// 14 of acc is not read), then 64 x the wave's largest Newton step count, the
+// largest NR error and the largest step count; lanes 0..16 write out[0..16].
+// The sums are a reduce-scatter in a fixed order: inside each row of 16 lanes
+// the pairs (row mirror, half-row mirror, quad reversal, quad swap) each
+// exchange half of what they hold, so a lane ends with one value summed over
+// its row, and lane swaps add the four rows.  ~120 instructions for the 14
+// sums against ~300 for one full reduction per value.
+__device__ __forceinline__ void wave_partial_row(const double (&acc)[15], double max_err, double max_steps,
+                                                 double* out) {
+  const int lane = threadIdx.x & 63;
+  double lo8[8], hi8[8], s8[8];
 #pragma unroll
-  for (int w = 0; w < W; w++) v[w] = wave_reduce<false>(v[w]);
-#pragma unroll
-  for (int w = 0; w < M; w++) vmax[w] = wave_reduce<true>(vmax[w]);
-  if ((threadIdx.x & 63) == 63) {
-#pragma unroll
-    for (int w = 0; w < W; w++) out[w] = v[w];
-#pragma unroll
-    for (int w = 0; w < M; w++) out[W + w] = vmax[w];
+  for (int j = 0; j < 8; j++) {
+    lo8[j] = acc[j];
+    hi8[j] = j + 8 < 14 ? acc[j + 8] : 0.0;
   }
+  pair_halve<0x140>(lo8, hi8, lane & 8, s8);  // row_mirror: lanes l, 15 - l
+  double lo4[4], hi4[4], s4[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    lo4[j] = s8[j];
+    hi4[j] = s8[j + 4];
+  }
+  pair_halve<0x141>(lo4, hi4, lane & 4, s4);  // row_half_mirror: l, l ^ 7 within 8
+  double lo2[2] = {s4[0], s4[1]}, hi2[2] = {s4[2], s4[3]}, s2[2];
+  pair_halve<0x1B>(lo2, hi2, lane & 2, s2);  // quad_perm [3,2,1,0]: l, l ^ 3
+  double lo1[1] = {s2[0]}, hi1[1] = {s2[1]}, s1[1];
+  pair_halve<0xB1>(lo1, hi1, lane & 1, s1);  // quad_perm [1,0,3,2]: l, l ^ 1
+  const double sum = cross_rows<true>(s1[0]);  // value lane % 16, over the wave
+  const double me = wave_max_all(max_err), ms = wave_max_all(max_steps);
+  const double val = lane < 14 ? sum : lane == 14 ? 64.0 * ms : lane == 15 ? me : ms;
+  if (lane < 17) out[lane] = val;
 }
 
 __device__ __forceinline__ void load_R(const double* __restrict__ R, long n, long v, double r[9]) {
@@ -609,31 +654,29 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
       double it[kSlips], r9[9];
       load_R(R, n, v, r9);
       if (UNIFORM_TAU) {
-        // every slip system of the voxel has the same CRSS (one load)
+        // every slip system of the voxel has the same CRSS (one load) and
+        // the same reference slip rate (the host checks both)
         const double t = 1.0 / crss[v];
 #pragma unroll
         for (int k = 0; k < kSlips; k++) it[k] = t;
       } else {
         load_inv_tau(crss, n, v, it);
       }
-      double s[5];
-#pragma unroll
-      for (int i = 0; i < 5; i++) s[i] = sigma[i * n + v];
-      // yield-surface rescale, MaterialGrid.cpp:954-959
+      // yield-surface rescale, MaterialGrid.cpp:954-959, of the stress in
+      // the crystal frame (the solve's start point)
       double mrss, sc[5];
       {
-        double Q[25];
+        double Q[25], s0[5];
+#pragma unroll
+        for (int i = 0; i < 5; i++) s0[i] = sigma[i * n + v];
         cf::dev_map(r9, Q);
-        cf::rot(Q, s, sc);
-        mrss = cf::max_rss(sc, it);
+        cf::rot(Q, s0, sc);
+        mrss = cf::max_rss<UNIFORM_TAU>(sc, it);
       }
       if (!(mrss >= 1e-10)) acc[13] += 1.0;
       const double imrss = rcp_nr(mrss);
 #pragma unroll
-      for (int i = 0; i < 5; i++) {
-        s[i] /= mrss;
-        sc[i] *= imrss;  // the crystal-frame start point of the solve
-      }
+      for (int i = 0; i < 5; i++) sc[i] *= imrss;
       // q = L0^-1 lambda + edot + d~ ; lambda and d~ are re-read after the solve
       double qv[5];
 #pragma unroll
@@ -646,20 +689,29 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
       }
       double err = 0.0;
       int steps = 0;
+      double G[15];
       {
         double P[25], qc[5], A0[15];
         cf::inverse_map(r9, P);
         cf::rot_t(P, qv, qc);
         cf::congruence_t(P, C.L0inv, A0);
-        cf::constrained_nr<NE>(s, sc, r9, P, it, C, qc, A0, 5e-4, 1e9, 100, &err, &steps);
+        cf::gram(P, G);
+        cf::constrained_nr<NE, UNIFORM_TAU && (NE > 0)>(sc, G, it, C, qc, A0, 5e-4, 1e9, 100, &err, &steps);
       }
-      double e[5];
+      // the solution in the sample frame, s = P sc = Q^T G sc, and its strain
+      // rate from the crystal frame, e = Q^T ec
+      double s[5], e[5];
       {
-        double Q[25], sc[5], ec[5];
+        // the orientation again (one more read of it, not 18 registers live
+        // through the solve): through a pointer the compiler cannot match
+        // with the first read
+
+        double Q[25], ec[5], gs[5];
         cf::dev_map(r9, Q);
-        cf::rot(Q, s, sc);
-        cf::apply_constitutive<NE>(sc, it, C, ec);
+        cf::apply_constitutive<NE, UNIFORM_TAU && (NE > 0)>(sc, it, C, ec);
         cf::rot_t(Q, ec, e);
+        cf::symv(G, sc, gs);
+        cf::rot_t(Q, gs, s);
       }
       // re-read d~ and lambda here rather than keep them live through the solve
       __asm__ volatile("" ::: "memory");
@@ -708,8 +760,7 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
     // lane efficiency of the Newton loop: a wave runs as many steps as its
     // slowest lane, the wave's maximum of the steps (slot 14 is 0 in every
     // lane before the reduction and 64 x that maximum after it)
-    wave_sum<15>(acc, mx, partial + blk * kPartialStride);
-    if ((threadIdx.x & 63) == 63) partial[blk * kPartialStride + 14] = 64.0 * mx[1];
+    wave_partial_row(acc, mx[0], mx[1], partial + blk * kPartialStride);
   }
   if constexpr (FUSED) {  // phi row: forward z transforms from LDS
     __syncthreads();
@@ -1015,12 +1066,15 @@ __global__ void k_constrained_batch(long n, const double* __restrict__ R, const 
     for (int i = 0; i < 5; i++) q[i] = Ll[i] + C.edot[i] + d[i];
     cf::rot_t(P, q, qc);
     cf::congruence_t(P, C.L0inv, A0);
-    double Q[25], sc[5];
+    double Q[25], sc[5], G[15], gs[5];
+    cf::gram(P, G);
     cf::dev_map(r, Q);
     cf::rot(Q, s, sc);
     double e = err[v];
     int st = 0;
-    cf::constrained_nr<0>(s, sc, r, P, it, C, qc, A0, 5e-4, 1e9, 100, &e, &st);
+    cf::constrained_nr<0>(sc, G, it, C, qc, A0, 5e-4, 1e9, 100, &e, &st);
+    cf::symv(G, sc, gs);
+    cf::rot_t(Q, gs, s);
     for (int i = 0; i < 5; i++) s_out[5 * v + i] = s[i];
     err[v] = e;
     steps[v] = st;
