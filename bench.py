@@ -183,7 +183,8 @@ def cpu_baseline(V, args, edot):
 def batched_workload(V, args):
     """Uneven RVEs advancing together: edges drawn from {16,24,32,48,64,96},
     grains log-uniform in 8-512, n from {5,10,20}, each its own random unit
-    traceless strain rate; FP32 FFT + FP64 Newton; tuned FFT block shapes;
+    traceless strain rate; FP32 FFT + FP64 Newton; FFT block shapes and
+    radix orders picked by the batch's autotuner;
     several strain steps of dt 0.005.  Timed once as one batch and once
     grid by grid through the single-grid entry point."""
     rng = np.random.default_rng(args.seed + 4)
@@ -207,8 +208,8 @@ def batched_workload(V, args):
 
     edots = np.array([c[3] for c in cases])
     voxels = sum(c[0] ** 3 for c in cases)
-    with V.Batch(tune=True) as b:
-        gs = make()  # warm-up pass: tunes the block shapes, loads the kernels
+    with V.Batch(tune="radix") as b:
+        gs = make()  # warm-up pass: tunes the schedules, loads the kernels
         b.RunSingleStrainStep(gs, edots, args.max_iter, args.dt, args.batch_tol)
         for g in gs:
             g.close()
@@ -220,6 +221,8 @@ def batched_workload(V, args):
             iters += sum(inf.iterations * c[0] ** 3 for inf, c in zip(b.last_infos, cases))
         wall = time.perf_counter() - t0
         shapes = sorted({(c[0], tuple(b.block_shape(g, p) for p in range(3))) for g, c in zip(gs, cases)})
+        radix = sorted({(c[0], tuple(".".join(map(str, b.radix_plan(g, p))) for p in range(3)))
+                        for g, c in zip(gs, cases)})
         for g in gs:
             g.close()
     gs = make()
@@ -237,6 +240,7 @@ def batched_workload(V, args):
         "rve_steps_per_s": rve_steps / wall, "voxel_iters_per_s": iters / wall, "wall_s": wall,
         "one_by_one_rve_steps_per_s": rve_steps / wall_seq, "speedup_vs_one_by_one": wall_seq / wall,
         "tuned_block_shapes_zyx": {str(e): list(sh) for e, sh in shapes},
+        "tuned_radix_plans_zyx": {str(e): list(r) for e, r in radix},
     }
 
 

@@ -261,13 +261,22 @@ int vpfft_grid_set_block_shape(void* grid, int precision, int pass, int block);
  * the caller. */
 int vpfft_batch_create(void** out_batch);
 int vpfft_batch_destroy(void* batch);
-/* enable = 1: the FFT block shape of each (precision, pass, grid shape) is
- * timed over the candidates once and the fastest kept (the arithmetic does
- * not depend on it); 0: fixed defaults. */
-int vpfft_batch_set_tuning(void* batch, int enable);
+/* The FFT schedule autotuner.  level 1: the block shape of each (precision,
+ * pass, grid shape) is timed over the candidates once, on the batch's own
+ * data, and the fastest kept (the arithmetic does not depend on it, so the
+ * bits do not change); level 2: block shapes and the radix order of the
+ * pass's axis (the stage sequence, e.g. 96 = 8.4.3 or 4.8.3 or 3.4.8),
+ * which changes the rounding of the transforms (results then agree with the
+ * default schedule to rounding, not bit for bit); 0: fixed defaults.
+ * Changing the level forgets earlier choices.  No reference counterpart (the
+ * reference's FFTW plans are made once, MaterialGrid.cpp:341-372). */
+int vpfft_batch_set_tuning(void* batch, int level);
 /* The block shape (pencils per block) a batch uses for grid's pass
  * (0 = z, 1 = y, 2 = x). */
 int vpfft_batch_block_shape(void* batch, const void* grid, int pass, int* block_out);
+/* The radix stages a batch uses for grid's pass (0 = z, 1 = y, 2 = x), in
+ * the order they run: radices[0..*nstages) (at most 12). */
+int vpfft_batch_radix_plan(void* batch, const void* grid, int pass, int* radices, int* nstages);
 /* One strain step of count grids.  edot: count x 5; dt: count time steps
  * (dt <= 0: no orientation / hardening update for that grid, as in
  * vpfft_run_strain_step); avg_stress_out: count x 5 (may be NULL); infos:

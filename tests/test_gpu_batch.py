@@ -1,6 +1,7 @@
 """Batched strain steps (vpfft_batch_*): many grids of uneven sizes advance
 together.  Each grid's result must be the same bits alone and in any batch,
-with or without block-shape tuning, and match the oracle."""
+with or without block-shape tuning, agree to rounding when the tuner also
+picks radix orders, and match the oracle either way."""
 import numpy as np
 import pytest
 
@@ -68,6 +69,43 @@ def test_batch_tuned_block_shapes_give_the_same_bits(gpu, vpfft):
     print("default shapes", shapes_plain, "tuned", shapes_tuned, flush=True)
 
 
+def test_batch_tuned_radix_orders_agree_to_rounding(gpu, vpfft):
+    """Level-2 tuning also picks each axis's radix order, which changes the
+    transforms' rounding only: same iteration counts, <sigma> and fields
+    within 1e-10 of the default schedule for FP64 grids (1e-6 for mixed
+    ones); every chosen plan is a factorisation of its axis into the
+    supported radices."""
+    print("batch tuned radix orders vs default schedule", flush=True)
+    idx = [0, 1, 3]
+    plain, st_plain, _ = _run_batch(vpfft, MIX, idx, 2)
+    grids = _grids(vpfft, [MIX[i] for i in idx], seeds=idx)
+    edots = np.array([random_unit_edot(100 + i) for i in idx])
+    tuned = []
+    with vpfft.Batch(tune="radix") as b:
+        for s in range(2):
+            avg = b.RunSingleStrainStep(grids, edots, 40, 0.005, 1e-6)
+            tuned.append((avg.copy(), [inf.iterations for inf in b.last_infos]))
+        for g, i in zip(grids, idx):
+            dims = MIX[i][0]
+            for p, n in zip(range(3), (dims[2], dims[1], dims[0])):
+                stages = b.radix_plan(g, p)
+                assert int(np.prod(stages)) == n and set(stages) <= {2, 3, 4, 5, 7, 8}, (dims, p, stages)
+            print(dims, "radix plans z/y/x:", [b.radix_plan(g, p) for p in range(3)], flush=True)
+    st_tuned = [_state(g) for g in grids]
+    for g in grids:
+        g.close()
+    # FP64 grids to 1e-10; a mixed grid's FP32 transforms round at ~1e-7, so
+    # its <sigma> moves by ~1e-9 (bar 1e-6)
+    bars = [1e-10 if MIX[i][3] == 0 else 1e-6 for i in idx]
+    for s in range(2):
+        assert tuned[s][1] == plain[s][1]
+        for j, bar in enumerate(bars):
+            assert rel_err(tuned[s][0][j], plain[s][0][j]) < bar, (s, j)
+    for a, b, bar in zip(st_plain, st_tuned, bars):
+        for k in a:
+            assert rel_err(b[k], a[k]) < bar, k
+
+
 def test_batch_of_one_matches_single_grid_path(gpu, vpfft):
     print("batch of one vs vpfft_run_strain_step", flush=True)
     dims = (24, 24, 24)
@@ -91,13 +129,14 @@ def test_batch_of_one_matches_single_grid_path(gpu, vpfft):
         assert rel_err(outs[1][k], outs[0][k]) < 1e-10
 
 
-def test_batch_matches_oracle(gpu, vpfft, oracle):
-    print("batch of 3 vs the oracle", flush=True)
+@pytest.mark.parametrize("tune", [False, "radix"])
+def test_batch_matches_oracle(gpu, vpfft, oracle, tune):
+    print(f"batch of 3 vs the oracle (tuning {tune})", flush=True)
     cases = [((16, 16, 16), 8, 10, 0), ((12, 18, 10), 5, 5, 0), ((20, 16, 12), 6, 20, 0)]
     grids = _grids(vpfft, cases, seed0=70)
     edots = np.array([random_unit_edot(200 + j) for j in range(3)])
     states = [dict(R=g.Orientation(), crss=g.CRSS(), acc=g.AccumulatedShear()) for g in grids]
-    with vpfft.Batch() as b:
+    with vpfft.Batch(tune=tune) as b:
         avg = b.RunSingleStrainStep(grids, edots, 30, 0.005, 1e-6)
         infos = b.last_infos
     for j, (dims, grains, nexp, prec) in enumerate(cases):

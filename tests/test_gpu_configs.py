@@ -141,11 +141,12 @@ def _uneven_cases(count, seed):
 def test_64_uneven_rves_in_one_batch(gpu, vpfft, oracle):
     """64 independent RVEs, edges drawn from {16,24,32,48,64,96}, grains
     log-uniform in 8-512, n from {5,10,20}, each its own unit strain rate;
-    FP32 FFT with FP64 Newton, tuned FFT block shapes, 5 steps of dt=0.005.
-    Every RVE's <sigma> after every step is the same bits in the batch of 64
-    and in a batch of its own; the RVEs of edge 16 and 24 are checked against
-    the FP64 oracle after the first step (<sigma> within 1e-5, the mixed
-    form's bar)."""
+    FP32 FFT with FP64 Newton, FFT schedules (block shapes and radix orders)
+    picked by the autotuner, 5 steps of dt=0.005.  Every RVE's <sigma> after
+    every step is the same bits in the batch of 64 and in a batch of its own
+    (one Batch, so both runs use the schedules it tuned); the RVEs of edge 16
+    and 24 are checked against the FP64 oracle after the first step (<sigma>
+    within 1e-5, the mixed form's bar)."""
     cases = _uneven_cases(64, 7)
     steps, max_iter, tol, dt = 5, 100, 1e-5, 0.005
     micro = [polycrystal(None, gr, sd, (e, e, e))[0] for (e, gr, _, _, sd) in cases]
@@ -160,15 +161,19 @@ def test_64_uneven_rves_in_one_batch(gpu, vpfft, oracle):
     edots = np.array([c[3] for c in cases])
     _say(f"batch of 64 uneven RVEs ({sum(c[0] ** 3 for c in cases)} voxels), {steps} steps")
     together = np.empty((steps, 64, 5))
-    with vpfft.Batch(tune=True) as b:
+    with vpfft.Batch(tune="radix") as b:
         gs = [make(i) for i in range(64)]
         for s in range(steps):
             together[s] = b.RunSingleStrainStep(gs, edots, max_iter, dt, tol)
             assert all(inf.status == 0 for inf in b.last_infos)
+        plans = {c[0]: [b.radix_plan(g, p) for p in range(3)] for g, c in zip(gs, cases)}
+        _say(f"tuned radix plans (z, y, x) by edge: {plans}")
+        for e, pl in plans.items():
+            for stages in pl:
+                assert int(np.prod(stages)) == e and set(stages) <= {2, 3, 4, 5, 7, 8}
         for g in gs:
             g.close()
-    _say("each RVE in a batch of its own")
-    with vpfft.Batch(tune=True) as b:
+        _say("each RVE in a batch of its own")
         for i in range(64):
             g = make(i)
             for s in range(steps):

@@ -126,6 +126,7 @@ EXPORTED = [
     "vpfft_solve_taylor_batch", "vpfft_generate_voronoi", "vpfft_last_error", "vpfft_version",
     "vpfft_grid_set_profiling", "vpfft_grid_kernel_times", "vpfft_kernel_name",
     "vpfft_batch_create", "vpfft_batch_destroy", "vpfft_batch_set_tuning", "vpfft_batch_block_shape",
+    "vpfft_batch_radix_plan",
     "vpfft_batch_run_strain_step", "vpfft_grid_set_block_shape", "vpfft_grid_bytes", "vpfft_grid_save_state",
     "vpfft_grid_restore_state", "vpfft_strain_rate_evolution_noalloc",
     "vpfft_solvers_taylor", "vpfft_solvers_constrained", "vpfft_solvers_sachs", "vpfft_solvers_apply",
@@ -186,6 +187,7 @@ def load_library(path: Optional[str] = None):
     lib.vpfft_batch_destroy.argtypes = [vp]
     lib.vpfft_batch_set_tuning.argtypes = [vp, C.c_int]
     lib.vpfft_batch_block_shape.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
+    lib.vpfft_batch_radix_plan.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     lib.vpfft_batch_run_strain_step.argtypes = [vp, C.POINTER(vp), C.c_int, dp, C.c_int, dp, C.c_double,
                                                 dp, C.POINTER(vpfft_step_info)]
     lib.vpfft_grid_bytes.argtypes = [C.POINTER(vpfft_grid_desc), C.POINTER(C.c_uint64)]
@@ -647,13 +649,16 @@ class Batch:
     """Advance several MaterialGrids (any sizes) through one strain step each,
     together (include/vpfft.h, vpfft_batch_*)."""
 
-    def __init__(self, tune: bool = False):
+    def __init__(self, tune=False):
+        """tune: False / 0 default schedules; True / 1 tuned block shapes (same
+        bits); 2 or "radix": tuned block shapes and radix orders."""
         self._lib = load_library()
         h = C.c_void_p()
         _check(self._lib.vpfft_batch_create(C.byref(h)))
         self._h = h
-        if tune:
-            _check(self._lib.vpfft_batch_set_tuning(self._h, 1))
+        level = 2 if tune == "radix" else int(tune)
+        if level:
+            _check(self._lib.vpfft_batch_set_tuning(self._h, level))
         self.last_infos: list = []
 
     def close(self):
@@ -677,6 +682,13 @@ class Batch:
         b = C.c_int()
         _check(self._lib.vpfft_batch_block_shape(self._h, grid._h, pass_, C.byref(b)))
         return b.value
+
+    def radix_plan(self, grid: "MaterialGrid", pass_: int) -> list:
+        """The radix stages of grid's pass (0 = z, 1 = y, 2 = x), in order."""
+        r = (C.c_int * 12)()
+        k = C.c_int()
+        _check(self._lib.vpfft_batch_radix_plan(self._h, grid._h, pass_, r, C.byref(k)))
+        return [r[i] for i in range(k.value)]
 
     def RunSingleStrainStep(self, grids, edots, NumMaxIterations: int, TimeStep: float,
                             ConvergenceEpsilon: float) -> np.ndarray:

@@ -828,6 +828,19 @@ struct BLaunch {
   int ids_at, off_at, count;  // slices of the id / offset pools
 };
 
+// The radix orders the tuner tries, as orders of preference for
+// make_axis_plan_ordered; the first is make_axis_plan's.
+static const int kRadixOrders[][6] = {{8, 4, 2, 3, 5, 7}, {4, 8, 2, 3, 5, 7}, {4, 2, 8, 3, 5, 7},
+                                      {2, 4, 8, 3, 5, 7}, {3, 5, 7, 8, 4, 2}, {7, 5, 3, 4, 8, 2}};
+constexpr int kNumRadixOrders = (int)(sizeof(kRadixOrders) / sizeof(kRadixOrders[0]));
+
+// An FFT pass's schedule for one grid shape: pencils per block and the
+// radix order of the pass's axis (an index into kRadixOrders).
+struct Sched {
+  int B = 0;
+  int order = 0;
+};
+
 struct Batch {
   hipStream_t st = nullptr;
   int cap = 0;
@@ -846,7 +859,9 @@ struct Batch {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int graph_iters = 4;
   int tune = 0;
-  std::map<std::tuple<int, int, int, int, int>, int> tuned;  // (prec, pass, nx, ny, nz) -> B
+  // tune: 0 default schedules, 1 block shapes (the bits do not change),
+  // 2 block shapes and radix orders (the rounding changes)
+  std::map<std::tuple<int, int, int, int, int>, Sched> tuned;  // (prec, pass, nx, ny, nz) -> schedule
   hipGraphExec_t gexec = nullptr;
   std::vector<unsigned long> gkey_grids;
   std::vector<int> gkey_vals;
@@ -955,44 +970,86 @@ static int enqueue_fft_launch(Batch* b, const BLaunch& L) {
   return VPFFT_OK;
 }
 
-// Block shape of an FFT pass for grid g: the tuned one when tuning is on.
-static int choose_B(Batch* b, Grid* g, FftPass pass, int job) {
-  if (!b->tune) return pass_shape(g->prec, pass, g->geom, 0).B;
+// The axis plan an FFT pass of job J reads (z: az, y: ay, x: ax).
+static AxisPlan* job_plan(FftJob* J, FftPass pass) { return pass == kPassZ ? &J->az : (pass == kPassY ? &J->ay : &J->ax); }
+static int pass_axis(FftPass pass) { return pass == kPassZ ? 2 : (pass == kPassY ? 1 : 0); }
+
+// The schedule of an FFT pass for grid g (job `job` of the step's table):
+// the default one, or the fastest of the candidates when tuning is on,
+// measured once per (precision, pass, grid shape) on the batch's stream with
+// this job's own data (the batched passes are the generic Stockham ones, so
+// every axis has a radix plan).  Radix orders are tried only at tune level 2.
+// The job's plan for the pass's axis is left at the choice.
+static int choose_sched(Batch* b, Grid* g, FftPass pass, int job, Sched* out) {
+  AxisPlan* jp = job_plan(b->hfft + job, pass);
+  const int axis = pass_axis(pass);
+  const int n = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
+  auto set_plan = [&](int order) {
+    if (order == 0) *jp = g->plan[axis];
+    else make_axis_plan_ordered(n, kRadixOrders[order], jp);
+  };
+  Sched best{pass_shape(g->prec, pass, g->geom, 0).B, 0};
+  if (!b->tune) {
+    set_plan(0);
+    *out = best;
+    return VPFFT_OK;
+  }
   const auto key = std::make_tuple(g->prec, (int)pass, g->nx, g->ny, g->nz);
   auto it = b->tuned.find(key);
-  if (it != b->tuned.end()) return it->second;
+  if (it != b->tuned.end()) {
+    set_plan(it->second.order);
+    *out = it->second;
+    return VPFFT_OK;
+  }
   int cand[8];
   const int nc = pass_candidates(pass, cand, 8);
-  int best = pass_shape(g->prec, pass, g->geom, 0).B;
+  // distinct plans of the axis (orders giving the same stage sequence once)
+  std::vector<int> orders{0};
+  if (b->tune >= 2) {
+    std::vector<unsigned long long> codes{g->plan[axis].code};
+    for (int o = 1; o < kNumRadixOrders; o++) {
+      AxisPlan ap;
+      if (!make_axis_plan_ordered(n, kRadixOrders[o], &ap)) continue;
+      if (std::find(codes.begin(), codes.end(), ap.code) != codes.end()) continue;
+      codes.push_back(ap.code);
+      orders.push_back(o);
+    }
+  }
   float best_ms = 1e30f;
   // a one-job group at the end of the pools (the step's launches use the front)
   const int at = b->pool_cap - 4;
   b->hids[at] = job;
   b->hoff[at + 1] = 0;
-  for (int c = 0; c < nc; c++) {
-    const PassShape ps = pass_shape(g->prec, pass, g->geom, cand[c]);
-    if (!ps.ok) continue;
-    b->hoff[at + 2] = ps.blocks;
-    if (hipMemcpyAsync(b->dids + at, b->hids + at, sizeof(int), hipMemcpyHostToDevice, b->st) != hipSuccess ||
-        hipMemcpyAsync(b->doff + at + 1, b->hoff + at + 1, 2 * sizeof(long), hipMemcpyHostToDevice, b->st) !=
-            hipSuccess)
-      return -1;
-    BLaunch L{pass == kPassZ ? (int)L_ZF : (pass == kPassY ? (int)L_YF : (int)L_X), g->prec, ps.B, ps.E, 0, 0,
-              ps.blocks, ps.lds, at, at + 1, 1};
-    float ms = 0;
-    for (int rep = 0; rep < 4; rep++) {
-      if (rep == 1 && hipEventRecord(b->ev[0], b->st) != hipSuccess) return -1;
-      if (enqueue_fft_launch(b, L)) return -1;
-    }
-    if (hipEventRecord(b->ev[1], b->st) != hipSuccess || hipEventSynchronize(b->ev[1]) != hipSuccess) return -1;
-    (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = ps.B;
+  for (int order : orders) {
+    set_plan(order);
+    HIP_TRY(hipMemcpyAsync(b->dfft + job, b->hfft + job, sizeof(FftJob), hipMemcpyHostToDevice, b->st));
+    for (int c = 0; c < nc; c++) {
+      const PassShape ps = pass_shape(g->prec, pass, g->geom, cand[c]);
+      if (!ps.ok) continue;
+      b->hoff[at + 2] = ps.blocks;
+      HIP_TRY(hipMemcpyAsync(b->dids + at, b->hids + at, sizeof(int), hipMemcpyHostToDevice, b->st));
+      HIP_TRY(hipMemcpyAsync(b->doff + at + 1, b->hoff + at + 1, 2 * sizeof(long), hipMemcpyHostToDevice, b->st));
+      BLaunch L{pass == kPassZ ? (int)L_ZF : (pass == kPassY ? (int)L_YF : (int)L_X), g->prec, ps.B, ps.E, 0, 0,
+                ps.blocks, ps.lds, at, at + 1, 1};
+      float ms = 0;
+      for (int rep = 0; rep < 4; rep++) {
+        if (rep == 1) HIP_TRY(hipEventRecord(b->ev[0], b->st));
+        int rc = enqueue_fft_launch(b, L);
+        if (rc) return rc;
+      }
+      HIP_TRY(hipEventRecord(b->ev[1], b->st));
+      HIP_TRY(hipEventSynchronize(b->ev[1]));
+      (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = Sched{ps.B, order};
+      }
     }
   }
+  set_plan(best.order);
   b->tuned[key] = best;
-  return best;
+  *out = best;
+  return VPFFT_OK;
 }
 
 // Group the grids of a step into launches and upload the tables.
@@ -1031,9 +1088,16 @@ static int batch_plan(Batch* b, Grid** gs, int count, const double* dt, const st
     std::map<std::tuple<int, int, int>, std::vector<int>> fg;
     for (int j = 0; j < count; j++) {
       if (!active[j]) continue;
-      const int B = choose_B(b, gs[j], pass, j);
-      if (B < 0) return set_error(VPFFT_E_DEVICE, "FFT block-shape tuning failed");
-      const PassShape ps = pass_shape(gs[j]->prec, pass, gs[j]->geom, B);
+      Sched sc;
+      if (kind == L_YI || kind == L_ZI) {  // the inverse passes take the forward pass's schedule
+        const auto key = std::make_tuple(gs[j]->prec, (int)pass, gs[j]->nx, gs[j]->ny, gs[j]->nz);
+        auto it = b->tuned.find(key);
+        sc = (b->tune && it != b->tuned.end()) ? it->second : Sched{pass_shape(gs[j]->prec, pass, gs[j]->geom, 0).B, 0};
+      } else {
+        int rc = choose_sched(b, gs[j], pass, j, &sc);
+        if (rc) return rc;
+      }
+      const PassShape ps = pass_shape(gs[j]->prec, pass, gs[j]->geom, sc.B);
       fg[std::make_tuple(gs[j]->prec, ps.B, ps.E)].push_back(j);
     }
     for (auto& kv : fg) add_group(kind, std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), 0, 0,
@@ -1050,6 +1114,8 @@ static int batch_plan(Batch* b, Grid** gs, int count, const double* dt, const st
   b->launches.push_back(BLaunch{-1, 0, 0, 0, 0, 0, 0, 0, ids_at, off_at, nact});
   ids_at += nact;
   if (ids_at > b->pool_cap - 8 || off_at > b->pool_cap - 8) return set_error(VPFFT_E_INVALID, "batch plan overflow");
+  // the job table with each grid's chosen radix plans
+  HIP_TRY(hipMemcpyAsync(b->dfft, b->hfft, sizeof(FftJob) * count, hipMemcpyHostToDevice, b->st));
   HIP_TRY(hipMemcpyAsync(b->dids, b->hids, sizeof(int) * ids_at, hipMemcpyHostToDevice, b->st));
   HIP_TRY(hipMemcpyAsync(b->doff, b->hoff, sizeof(long) * std::max(off_at, 1), hipMemcpyHostToDevice, b->st));
   return VPFFT_OK;
@@ -2013,9 +2079,12 @@ int vpfft_batch_destroy(void* batch) {
   return VPFFT_OK;
 }
 
-int vpfft_batch_set_tuning(void* batch, int enable) {
+int vpfft_batch_set_tuning(void* batch, int level) {
   if (!batch) return set_error(VPFFT_E_INVALID, "null batch");
-  static_cast<vpfft::Batch*>(batch)->tune = enable ? 1 : 0;
+  if (level < 0 || level > 2) return set_error(VPFFT_E_INVALID, "tuning level must be 0, 1 or 2");
+  vpfft::Batch* b = static_cast<vpfft::Batch*>(batch);
+  if (level != b->tune) b->tuned.clear();
+  b->tune = level;
   return VPFFT_OK;
 }
 
@@ -2025,8 +2094,24 @@ int vpfft_batch_block_shape(void* batch, const void* grid, int pass, int* block_
   if (!b || !g || !block_out || pass < 0 || pass > 2) return set_error(VPFFT_E_INVALID, "bad argument");
   const auto key = std::make_tuple(g->prec, pass, g->nx, g->ny, g->nz);
   auto it = b->tuned.find(key);
-  *block_out = (b->tune && it != b->tuned.end()) ? it->second
+  *block_out = (b->tune && it != b->tuned.end()) ? it->second.B
                                                  : vpfft::pass_shape(g->prec, (vpfft::FftPass)pass, g->geom, 0).B;
+  return VPFFT_OK;
+}
+
+int vpfft_batch_radix_plan(void* batch, const void* grid, int pass, int* radices, int* nstages) {
+  vpfft::Batch* b = static_cast<vpfft::Batch*>(batch);
+  const Grid* g = static_cast<const Grid*>(grid);
+  if (!b || !g || !radices || !nstages || pass < 0 || pass > 2) return set_error(VPFFT_E_INVALID, "bad argument");
+  const auto key = std::make_tuple(g->prec, pass, g->nx, g->ny, g->nz);
+  auto it = b->tuned.find(key);
+  const int order = (b->tune && it != b->tuned.end()) ? it->second.order : 0;
+  const int axis = pass == 0 ? 2 : (pass == 1 ? 1 : 0);  // pass 0 z, 1 y, 2 x
+  const int n = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
+  vpfft::AxisPlan ap;
+  if (!vpfft::make_axis_plan_ordered(n, vpfft::kRadixOrders[order], &ap)) return set_error(VPFFT_E_STATE, "no plan");
+  for (int s = 0; s < ap.nstages; s++) radices[s] = ap.radix[s];
+  *nstages = ap.nstages;
   return VPFFT_OK;
 }
 
