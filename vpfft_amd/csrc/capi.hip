@@ -646,7 +646,8 @@ static int strain_step(Grid* g, const double edot[5], int max_iter, double dt, d
   HIP_TRY(hipEventRecord(g->ev[0], g->st));
   // prologue: MaterialGrid.cpp:903-913
   kbegin(g, K_PROLOGUE);
-  HIP_TRY(launch_prologue(prec, n, g->R, g->crss, g->dC, g->phi, g->sigma, g->lambda, g->partial, g->dhealth, g->st));
+  HIP_TRY(launch_prologue(prec, g->crss_uniform, n, g->R, g->crss, g->dC, g->phi, g->sigma, g->lambda, g->partial,
+                          g->dhealth, g->st));
   kend(g, K_PROLOGUE);
   g->phi_prec = prec;
   kbegin(g, K_L0_FINALIZE);

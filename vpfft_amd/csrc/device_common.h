@@ -803,28 +803,65 @@ __device__ __forceinline__ void schmid(const double R[9], double m[kSlips][5]) {
   }
 }
 
-__device__ __forceinline__ double max_rss(const double s[5], const double m[kSlips][5], const double tau[kSlips]) {
+// Where a voxel's 12 Schmid tensors are read from: m(k, i) is component i
+// of slip k.  SchmidRegs: an array in registers.  SchmidSplit: slips below KR
+// in registers, the others in this wave's LDS rows (lane-major, so a read is
+// one conflict-free 64-lane access); the values are the same doubles either
+// way, only fewer registers are live through the Taylor solve.
+struct SchmidRegs {
+  const double (*m)[5];
+  __device__ __forceinline__ double operator()(int k, int i) const { return m[k][i]; }
+};
+template <int KR>
+struct SchmidSplit {
+  double reg[KR > 0 ? KR : 1][5];
+  const double* lds;  // [(k - KR) * 5 + i][64]
+  int lane;
+  __device__ __forceinline__ double operator()(int k, int i) const {
+    return k < KR ? reg[k < KR ? k : 0][i] : lds[((k - KR) * 5 + i) * 64 + lane];
+  }
+};
+
+// m_k . s in dot5's order
+template <typename MS>
+__device__ __forceinline__ double mdot(const MS& m, int k, const double* s) {
+#pragma clang fp contract(off)
+  double r = 0.0;
+#pragma unroll
+  for (int i = 0; i < 5; i++) r += m(k, i) * s[i];
+  return r;
+}
+
+template <typename MS>
+__device__ __forceinline__ double max_rss_m(const double s[5], const MS& m, const double tau[kSlips]) {
 #pragma clang fp contract(off)
   double mx = 0.0;
 #pragma unroll
   for (int k = 0; k < kSlips; k++) {
-    const double v = fabs(dot5(m[k], s) / tau[k]);
+    const double v = fabs(mdot(m, k, s) / tau[k]);
     mx = v > mx ? v : mx;
   }
   return mx;
 }
+__device__ __forceinline__ double max_rss(const double s[5], const double m[kSlips][5], const double tau[kSlips]) {
+  return max_rss_m(s, SchmidRegs{m}, tau);
+}
 
-__device__ __forceinline__ void sachs(const double m[kSlips][5], const double tau[kSlips], const double edot[5],
-                                      double out[5]) {
+template <typename MS>
+__device__ __forceinline__ void sachs_m(const MS& m, const double tau[kSlips], const double edot[5], double out[5]) {
 #pragma clang fp contract(off)
   double mx = 0.0;
 #pragma unroll
   for (int k = 0; k < kSlips; k++) {
-    const double v = fabs(dot5(edot, m[k]) / tau[k]);
+    const double v = fabs(mdot(m, k, edot) / tau[k]);
     mx = v > mx ? v : mx;
   }
 #pragma unroll
   for (int i = 0; i < 5; i++) out[i] = edot[i] / mx;
+}
+__device__ __forceinline__ void sachs(const double m[kSlips][5], const double tau[kSlips], const double edot[5],
+                                      double out[5]) {
+  sachs_m(SchmidRegs{m}, tau, edot, out);
 }
 
 // Complete-pivoting LU solve; A column-major.  Pivot scan column-major with a
@@ -1014,7 +1051,8 @@ __device__ __forceinline__ void secant_modulus(const double m[kSlips][5], const 
 }
 
 // Solvers::SolveConstitutiveEquations (Solvers.cpp:19-101), operation for operation.
-__device__ __forceinline__ void taylor_nr(const double init[5], const double m[kSlips][5], const double tau[kSlips],
+template <typename MS>
+__device__ __forceinline__ void taylor_nr_m(const double init[5], const MS& m, const double tau[kSlips],
                                           const StepConst& C, double eps, int max_it, double out[5],
                                           int* steps_out = nullptr) {
 #pragma clang fp contract(off)
@@ -1027,13 +1065,16 @@ __device__ __forceinline__ void taylor_nr(const double init[5], const double m[k
   int steps = 0;
   const double max_rss = C.imposed_max_rss;
   while (rem > 0) {
+    // the Schmid tensors held in LDS are read again each step rather than
+    // kept in registers across the loop
+    __asm__ volatile("" ::: "memory");
     double rss[kSlips];
     bool ok;
     do {
       ok = true;
 #pragma unroll
       for (int k = 0; k < kSlips; k++) {
-        double r = div_cr(dot5(m[k], cur), tau[k], rtau[k]);
+        double r = div_cr(mdot(m, k, cur), tau[k], rtau[k]);
         if (fabs(r) < 1e-10) r = 0.0;
         if (fabs(r) > max_rss) ok = false;
         rss[k] = r;
@@ -1059,14 +1100,14 @@ __device__ __forceinline__ void taylor_nr(const double init[5], const double m[k
       const int n = C.uniform_n ? C.nexp[0] : C.nexp[k];
       const double g = (n - 1 > 0) ? C.gdot0[k] * ipow_cr(fabs(rss[k]), n - 1) : C.gdot0[k];
 #pragma unroll
-      for (int i = 0; i < 5; i++) res[i] += m[k][i] * g * rss[k];
+      for (int i = 0; i < 5; i++) res[i] += m(k, i) * g * rss[k];
       // J(i,j) and J(j,i) receive bit-identical increments (m_i m_j == m_j m_i),
       // so the lower triangle is computed and mirrored
 #pragma unroll
       for (int j = 0; j < 5; j++)
 #pragma unroll
         for (int i = j; i < 5; i++) {
-          const double t = m[k][i] * m[k][j] * (double)n * g;
+          const double t = m(k, i) * m(k, j) * (double)n * g;
           J[j * 5 + i] -= div_cr(t, tau[k], rtau[k]);
         }
     }
@@ -1095,6 +1136,12 @@ __device__ __forceinline__ void taylor_nr(const double init[5], const double m[k
 #pragma unroll
   for (int i = 0; i < 5; i++) out[i] = nw[i];
   if (steps_out) *steps_out = steps;
+}
+
+__device__ __forceinline__ void taylor_nr(const double init[5], const double m[kSlips][5], const double tau[kSlips],
+                                          const StepConst& C, double eps, int max_it, double out[5],
+                                          int* steps_out = nullptr) {
+  taylor_nr_m(init, SchmidRegs{m}, tau, C, eps, max_it, out, steps_out);
 }
 
 // Solvers::SolveConstrainedConstitutiveEquations (Solvers.cpp:111-222) in the

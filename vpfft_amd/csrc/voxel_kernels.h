@@ -85,8 +85,10 @@ enum HealthField {
 const char* health_kernel_name(int k);
 const char* health_field_name(int f);
 
-hipError_t launch_prologue(int prec, long n, const double* R, const double* crss, const StepConst* C, void* field,
-                           double* sigma, double* lambda, double* partial, HealthRecord* health, hipStream_t st);
+// uniform_crss: every slip system of a voxel has the same CRSS (crss[0][v] is read)
+hipError_t launch_prologue(int prec, bool uniform_crss, long n, const double* R, const double* crss,
+                           const StepConst* C, void* field, double* sigma, double* lambda, double* partial,
+                           HealthRecord* health, hipStream_t st);
 hipError_t launch_L0_finalize(long n, int nblocks, const double* partial, StepConst* C, GreenConst<double>* g64,
                               GreenConst<float>* g32, PrologueRecord* rec, HealthRecord* health, hipStream_t st);
 hipError_t launch_health_reset(HealthRecord* health, hipStream_t st);

@@ -238,22 +238,48 @@ __device__ __forceinline__ long cube_voxel(long blk, int tid, int nx, int ny, in
 // InitializeComputation for voxel v (MaterialGrid.cpp:153-210): sigma,
 // lambda and the polarization get the Taylor stress; the degenerate count and
 // the Taylor Newton steps go to acc[0], acc[1].
-template <typename T>
+// The prologue holds slips 0..kPrologueRegSlips-1 of the voxel's Schmid
+// tensors in registers and the other seven in LDS (17.5 KB per wave, two
+// 256-thread blocks per CU), so the Taylor solve runs at two waves per SIMD.
+constexpr int kPrologueRegSlips = 5;
+constexpr int kPrologueLdsDoubles = (kSlips - kPrologueRegSlips) * 5 * 64;  // per wave
+
+template <typename T, bool UNI>
 __device__ __forceinline__ void prologue_voxel(long v, long n, const double* __restrict__ R,
                                                const double* __restrict__ crss, const StepConst& C,
                                                T* __restrict__ field, double* __restrict__ sigma,
                                                double* __restrict__ lambda, double (&acc)[2],
-                                               HealthRecord* __restrict__ health) {
-  double r[9], tau[kSlips], m[kSlips][5];
-  load_R(R, n, v, r);
+                                               HealthRecord* __restrict__ health, double* mlds) {
+  double tau[kSlips];
+  if (UNI) {  // every slip system of the voxel has the same CRSS
+    const double t = crss[v];
 #pragma unroll
-  for (int k = 0; k < kSlips; k++) tau[k] = crss[k * n + v];
+    for (int k = 0; k < kSlips; k++) tau[k] = t;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kSlips; k++) tau[k] = crss[k * n + v];
+  }
   // bit-faithful: Schmid tensors, Sachs guess (Solvers.cpp:230-243), max RSS
   // rescale (MaterialGrid.cpp:175-190), Taylor NR (Solvers.cpp:19-101)
-  faithful::schmid(r, m);
+  const int lane = threadIdx.x & 63;
+  faithful::SchmidSplit<kPrologueRegSlips> m;
+  {
+    double r[9], mm[kSlips][5];
+    load_R(R, n, v, r);
+    faithful::schmid(r, mm);
+#pragma unroll
+    for (int k = 0; k < kSlips; k++)
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        if (k < kPrologueRegSlips) m.reg[k < kPrologueRegSlips ? k : 0][i] = mm[k][i];
+        else mlds[((k - kPrologueRegSlips) * 5 + i) * 64 + lane] = mm[k][i];
+      }
+  }
+  m.lds = mlds;
+  m.lane = lane;
   double guess[5];
-  faithful::sachs(m, tau, C.edot, guess);
-  const double mrss = faithful::max_rss(guess, m, tau);
+  faithful::sachs_m(m, tau, C.edot, guess);
+  const double mrss = faithful::max_rss_m(guess, m, tau);
   if (mrss > 1.1) {
 #pragma unroll
     for (int i = 0; i < 5; i++) guess[i] /= mrss;
@@ -261,7 +287,7 @@ __device__ __forceinline__ void prologue_voxel(long v, long n, const double* __r
   if (!(mrss >= 1e-10)) acc[0] += 1.0;  // degenerate (the reference exits here)
   double s[5];
   int tsteps = 0;
-  faithful::taylor_nr(guess, m, tau, C, 5e-4, 1000, s, &tsteps);
+  faithful::taylor_nr_m(guess, m, tau, C, 5e-4, 1000, s, &tsteps);
   acc[1] += (double)tsteps;
 #pragma unroll
   for (int i = 0; i < 5; i++) {
@@ -309,13 +335,12 @@ __device__ __forceinline__ void secant_voxel(long v, long n, const double* __res
 // 15 (degenerate voxels) and 16 (Taylor Newton steps) of row b, the secant
 // kernel's block b slots 0..14 (inverse secant moduli, packed lower) of row
 // nb + b, where nb is the number of blocks of either launch.
-template <typename T>
-__global__ void __launch_bounds__(kPointThreads) k_prologue(long n, const double* __restrict__ R,
-                                                            const double* __restrict__ crss,
-                                                            const StepConst* __restrict__ Cp, T* __restrict__ field,
-                                                            double* __restrict__ sigma, double* __restrict__ lambda,
-                                                            double* __restrict__ partial,
-                                                            HealthRecord* __restrict__ health) {
+template <typename T, bool UNI>
+__global__ void __launch_bounds__(kPointThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    k_prologue(long n, const double* __restrict__ R, const double* __restrict__ crss,
+               const StepConst* __restrict__ Cp, T* __restrict__ field, double* __restrict__ sigma,
+               double* __restrict__ lambda, double* __restrict__ partial, HealthRecord* __restrict__ health) {
+  __shared__ double mlds[kPointThreads / 64][kPrologueLdsDoubles];
   const StepConst& C = *Cp;
   double acc[2] = {0.0, 0.0};
   double mx[1] = {0.0};
@@ -323,7 +348,7 @@ __global__ void __launch_bounds__(kPointThreads) k_prologue(long n, const double
   for (long i = (long)blockIdx.x * kPointThreads + threadIdx.x; i < (long)pt_blocks(n) * kPointThreads;
        i += (long)gridDim.x * kPointThreads) {
     const long v = cube_voxel(i / kPointThreads, threadIdx.x, C.nx, C.ny, C.nz);
-    if (v < n) prologue_voxel<T>(v, n, R, crss, C, field, sigma, lambda, acc, health);
+    if (v < n) prologue_voxel<T, UNI>(v, n, R, crss, C, field, sigma, lambda, acc, health, mlds[threadIdx.x >> 6]);
   }
   block_sum<2>(acc, mx, partial + (long)blockIdx.x * 18 + 15);
 }
@@ -970,14 +995,18 @@ __global__ void __launch_bounds__(kPointThreads) k_epilogue(long n, double* __re
 // job's partial rows and their reduction order depend on that job alone.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(kPointThreads) k_prologue_b(VoxGroup grp) {
+__global__ void __launch_bounds__(kPointThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    k_prologue_b(VoxGroup grp) {
+  __shared__ double mlds[kPointThreads / 64][kPrologueLdsDoubles];
   long local;
   const VoxJob J = vox_job(grp.jobs + vox_member(grp, blockIdx.x, &local));
   double acc[2] = {0.0, 0.0};
   double mx[1] = {0.0};
   const StepConst& C = *as_const_mem(J.C);
   const long v = cube_voxel(local, threadIdx.x, C.nx, C.ny, C.nz);
-  if (v < J.n) prologue_voxel<T>(v, J.n, J.R, J.crss, C, (T*)J.phi, J.sigma, J.lambda, acc, J.health);
+  if (v < J.n)
+    prologue_voxel<T, false>(v, J.n, J.R, J.crss, C, (T*)J.phi, J.sigma, J.lambda, acc, J.health,
+                             mlds[threadIdx.x >> 6]);
   block_sum<2>(acc, mx, J.partial + local * 18 + 15);
 }
 
@@ -1238,15 +1267,19 @@ hipError_t launch_solvers(int op, long n, const double* schmid, const double* an
 }
 
 // ---------------------------------------------------------------------------
-hipError_t launch_prologue(int prec, long n, const double* R, const double* crss, const StepConst* C, void* field,
-                           double* sigma, double* lambda, double* partial, HealthRecord* health, hipStream_t st) {
+hipError_t launch_prologue(int prec, bool uniform_crss, long n, const double* R, const double* crss,
+                           const StepConst* C, void* field, double* sigma, double* lambda, double* partial,
+                           HealthRecord* health, hipStream_t st) {
   const int nb = point_blocks(n);
-  if (prec == 0)
-    hipLaunchKernelGGL(k_prologue<double>, dim3(nb), dim3(kPointThreads), 0, st, n, R, crss, C, (double*)field,
-                       sigma, lambda, partial, health);
-  else
-    hipLaunchKernelGGL(k_prologue<float>, dim3(nb), dim3(kPointThreads), 0, st, n, R, crss, C, (float*)field,
-                       sigma, lambda, partial, health);
+#define PRO_LAUNCH(TT, UU)                                                                                        \
+  hipLaunchKernelGGL((k_prologue<TT, UU>), dim3(nb), dim3(kPointThreads), 0, st, n, R, crss, C, (TT*)field, sigma, \
+                     lambda, partial, health)
+  if (prec == 0) {
+    if (uniform_crss) PRO_LAUNCH(double, true); else PRO_LAUNCH(double, false);
+  } else {
+    if (uniform_crss) PRO_LAUNCH(float, true); else PRO_LAUNCH(float, false);
+  }
+#undef PRO_LAUNCH
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_secant, dim3(nb), dim3(kPointThreads), 0, st, n, R, crss, sigma, C, partial, health);
