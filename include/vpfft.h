@@ -272,7 +272,10 @@ int vpfft_batch_destroy(void* batch);
  * reference's FFTW plans are made once, MaterialGrid.cpp:341-372). */
 int vpfft_batch_set_tuning(void* batch, int level);
 /* The block shape (pencils per block) a batch uses for grid's pass
- * (0 = z, 1 = y, 2 = x). */
+ * (0 = z, 1 = y, 2 = x); a negative value -n is the register form of an
+ * n-point axis (one thread per pencil, the whole pencil in registers: the
+ * default for the y pass of 16-, 24- and 32-point axes, and of 48- and
+ * 64-point ones in the mixed form). */
 int vpfft_batch_block_shape(void* batch, const void* grid, int pass, int* block_out);
 /* The radix stages a batch uses for grid's pass (0 = z, 1 = y, 2 = x), in
  * the order they run: radices[0..*nstages) (at most 12). */

@@ -981,6 +981,13 @@ static int pass_axis(FftPass pass) { return pass == kPassZ ? 2 : (pass == kPassY
 // this job's own data (the batched passes are the generic Stockham ones, so
 // every axis has a radix plan).  Radix orders are tried only at tune level 2.
 // The job's plan for the pass's axis is left at the choice.
+// The default schedule of a batched pass: the register form where the axis
+// has one, else the generic pass's default block shape.
+static Sched default_sched(const Grid* g, FftPass pass) {
+  const PassShape rf = pass_shape(g->prec, pass, g->geom, kRegForm);
+  return Sched{rf.ok ? rf.B : pass_shape(g->prec, pass, g->geom, 0).B, 0};
+}
+
 static int choose_sched(Batch* b, Grid* g, FftPass pass, int job, Sched* out) {
   AxisPlan* jp = job_plan(b->hfft + job, pass);
   const int axis = pass_axis(pass);
@@ -989,8 +996,10 @@ static int choose_sched(Batch* b, Grid* g, FftPass pass, int job, Sched* out) {
     if (order == 0) *jp = g->plan[axis];
     else make_axis_plan_ordered(n, kRadixOrders[order], jp);
   };
-  Sched best{pass_shape(g->prec, pass, g->geom, 0).B, 0};
-  if (!b->tune) {
+  Sched best = default_sched(g, pass);
+  // level 0, and level 1 for a register-form default (its arithmetic has no
+  // block shape to vary): the default
+  if (!b->tune || (b->tune == 1 && best.B < 0)) {
     set_plan(0);
     *out = best;
     return VPFFT_OK;
@@ -1002,8 +1011,10 @@ static int choose_sched(Batch* b, Grid* g, FftPass pass, int job, Sched* out) {
     *out = it->second;
     return VPFFT_OK;
   }
-  int cand[8];
-  const int nc = pass_candidates(pass, cand, 8);
+  int cand[9];
+  int nc = pass_candidates(pass, cand, 8);
+  // level 2 also weighs the register form against the generic pass
+  if (b->tune >= 2 && pass_shape(g->prec, pass, g->geom, kRegForm).ok) cand[nc++] = kRegForm;
   // distinct plans of the axis (orders giving the same stage sequence once)
   std::vector<int> orders{0};
   if (b->tune >= 2) {
@@ -1026,7 +1037,7 @@ static int choose_sched(Batch* b, Grid* g, FftPass pass, int job, Sched* out) {
     HIP_TRY(hipMemcpyAsync(b->dfft + job, b->hfft + job, sizeof(FftJob), hipMemcpyHostToDevice, b->st));
     for (int c = 0; c < nc; c++) {
       const PassShape ps = pass_shape(g->prec, pass, g->geom, cand[c]);
-      if (!ps.ok) continue;
+      if (!ps.ok || (ps.B < 0 && order != orders[0])) continue;  // the register form has no radix plan to vary
       b->hoff[at + 2] = ps.blocks;
       HIP_TRY(hipMemcpyAsync(b->dids + at, b->hids + at, sizeof(int), hipMemcpyHostToDevice, b->st));
       HIP_TRY(hipMemcpyAsync(b->doff + at + 1, b->hoff + at + 1, 2 * sizeof(long), hipMemcpyHostToDevice, b->st));
@@ -1093,7 +1104,7 @@ static int batch_plan(Batch* b, Grid** gs, int count, const double* dt, const st
       if (kind == L_YI || kind == L_ZI) {  // the inverse passes take the forward pass's schedule
         const auto key = std::make_tuple(gs[j]->prec, (int)pass, gs[j]->nx, gs[j]->ny, gs[j]->nz);
         auto it = b->tuned.find(key);
-        sc = (b->tune && it != b->tuned.end()) ? it->second : Sched{pass_shape(gs[j]->prec, pass, gs[j]->geom, 0).B, 0};
+        sc = (b->tune && it != b->tuned.end()) ? it->second : default_sched(gs[j], pass);
       } else {
         int rc = choose_sched(b, gs[j], pass, j, &sc);
         if (rc) return rc;
@@ -2095,8 +2106,7 @@ int vpfft_batch_block_shape(void* batch, const void* grid, int pass, int* block_
   if (!b || !g || !block_out || pass < 0 || pass > 2) return set_error(VPFFT_E_INVALID, "bad argument");
   const auto key = std::make_tuple(g->prec, pass, g->nx, g->ny, g->nz);
   auto it = b->tuned.find(key);
-  *block_out = (b->tune && it != b->tuned.end()) ? it->second.B
-                                                 : vpfft::pass_shape(g->prec, (vpfft::FftPass)pass, g->geom, 0).B;
+  *block_out = (b->tune && it != b->tuned.end()) ? it->second.B : vpfft::default_sched(g, (vpfft::FftPass)pass).B;
   return VPFFT_OK;
 }
 
@@ -2106,11 +2116,15 @@ int vpfft_batch_radix_plan(void* batch, const void* grid, int pass, int* radices
   if (!b || !g || !radices || !nstages || pass < 0 || pass > 2) return set_error(VPFFT_E_INVALID, "bad argument");
   const auto key = std::make_tuple(g->prec, pass, g->nx, g->ny, g->nz);
   auto it = b->tuned.find(key);
-  const int order = (b->tune && it != b->tuned.end()) ? it->second.order : 0;
+  const vpfft::Sched sc = (b->tune && it != b->tuned.end()) ? it->second : vpfft::default_sched(g, (vpfft::FftPass)pass);
   const int axis = pass == 0 ? 2 : (pass == 1 ? 1 : 0);  // pass 0 z, 1 y, 2 x
   const int n = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
+  if (sc.B < 0) {  // the register form: its compile-time radix chain
+    *nstages = vpfft::reg_form_radices(n, radices);
+    return VPFFT_OK;
+  }
   vpfft::AxisPlan ap;
-  if (!vpfft::make_axis_plan_ordered(n, vpfft::kRadixOrders[order], &ap)) return set_error(VPFFT_E_STATE, "no plan");
+  if (!vpfft::make_axis_plan_ordered(n, vpfft::kRadixOrders[sc.order], &ap)) return set_error(VPFFT_E_STATE, "no plan");
   for (int s = 0; s < ap.nstages; s++) radices[s] = ap.radix[s];
   *nstages = ap.nstages;
   return VPFFT_OK;

@@ -67,6 +67,14 @@ struct PassShape {
   bool ok;
 };
 enum FftPass { kPassZ = 0, kPassY = 1, kPassX = 2 };
+// The register form of a pass (one thread per pencil, the pencil in
+// registers): pass_shape(prec, pass, g, kRegForm) gives B = -(axis length)
+// when that length has one (batched y passes of 16, 24, 32 points, and 48
+// and 64 in FP32), else ok = false.
+constexpr int kRegForm = -1;
+bool reg_form_supported(int prec, FftPass pass, int n);
+// the register form's radix stages for length n, in the order they run; returns the count
+int reg_form_radices(int n, int* out);
 PassShape pass_shape(int prec, FftPass pass, const FftGeom& g, int B);
 // candidate B values of a pass (for the tuner), returns the count
 int pass_candidates(FftPass pass, int* out, int cap);

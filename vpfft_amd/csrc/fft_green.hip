@@ -27,6 +27,8 @@
 // operator G_eff(k) = (G(k) + G(-k))/2.  That is what the x pass applies.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device_common.h"
 #include "fft_green.h"
 #include "fft_regs.h"
@@ -1102,6 +1104,142 @@ __global__ void __launch_bounds__(NT) k_x_green_b(FftGroup grp) {
 }
 
 // ---------------------------------------------------------------------------
+// Register form of a pass for short axes: one thread per pencil, the whole
+// pencil in registers, a mixed-radix decimation-in-time transform unrolled
+// at compile time (N = R M: M-point transforms of the R interleaved
+// subsequences, twiddles W_N^(r k1), R-point transforms across them).  No
+// LDS and no barrier; the lanes of a wave take consecutive kz, so each of the
+// pencil's loads and stores is one contiguous run of the wave's 64 values.
+// Twiddles come from the axis's table (W_n^j) at compile-time offsets.
+// ---------------------------------------------------------------------------
+template <int N>
+constexpr int reg_radix() {
+  return N == 16 ? 4
+         : (N % 8 == 0 && N > 8) ? 8
+         : (N % 4 == 0 && N > 4) ? 4
+         : (N % 2 == 0 && N > 2) ? 2
+         : (N % 3 == 0 && N > 3) ? 3
+         : (N % 5 == 0 && N > 5) ? 5
+         : (N % 7 == 0 && N > 7) ? 7 : N;
+}
+
+// W_N^j = exp(-2 pi i j / N) evaluated at compile time: the angle reduced by
+// exact integer arithmetic to [0, pi/4] (quadrant, then the complement),
+// then Taylor series to x^23 (error below 1e-20 there).
+struct CxCS {
+  double c, s;
+};
+constexpr double cx_sin(double x) {
+  double t = x, r = x;
+  for (int i = 1; i <= 11; i++) {
+    t *= -x * x / (double)((2 * i) * (2 * i + 1));
+    r += t;
+  }
+  return r;
+}
+constexpr double cx_cos(double x) {
+  double t = 1.0, r = 1.0;
+  for (int i = 1; i <= 11; i++) {
+    t *= -x * x / (double)((2 * i - 1) * (2 * i));
+    r += t;
+  }
+  return r;
+}
+constexpr CxCS cx_twiddle(int j, int n) {  // cos and sin of 2 pi j / n
+  j %= n;
+  const int q = (4 * j) / n;  // quadrant
+  const int rr = 4 * j - q * n;  // angle within it: (rr / n) pi / 2
+  const double hp = 1.57079632679489661923;
+  double c0, s0;
+  if (2 * rr <= n) {
+    const double a = (double)rr / (double)n * hp;
+    c0 = cx_cos(a);
+    s0 = cx_sin(a);
+  } else {
+    const double a = (double)(n - rr) / (double)n * hp;
+    c0 = cx_sin(a);
+    s0 = cx_cos(a);
+  }
+  switch (q) {
+    case 0: return CxCS{c0, s0};
+    case 1: return CxCS{-s0, c0};
+    case 2: return CxCS{-c0, -s0};
+    default: return CxCS{s0, -c0};
+  }
+}
+template <int X, int N>
+struct TwC {  // W_N^X (forward sign: exp(-2 pi i X / N))
+  static constexpr CxCS v = cx_twiddle(X, N);
+};
+
+// compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
+template <int B, int E, typename F>
+__device__ __forceinline__ void sfor(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    sfor<B + 1, E>(f);
+  }
+}
+
+// transform of v (natural order in and out), sign -1 forward, +1 inverse
+template <typename T, int N>
+__device__ __forceinline__ void reg_fft(typename C2<T>::t (&v)[N], int sign) {
+  using C = typename C2<T>::t;
+  constexpr int R = reg_radix<N>();
+  if constexpr (R == N) {
+    dft<T, N>(v, sign);
+  } else {
+    constexpr int M = N / R;
+    C y[R][M];
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+      for (int m = 0; m < M; m++) y[r][m] = v[r + R * m];
+#pragma unroll
+    for (int r = 0; r < R; r++) reg_fft<T, M>(y[r], sign);
+    sfor<0, M>([&](auto k1c) {
+      constexpr int k1 = decltype(k1c)::value;
+      C t[R];
+      sfor<0, R>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        if constexpr (r * k1 == 0) {
+          t[r] = y[r][k1];
+        } else {
+          constexpr CxCS w = TwC<(r * k1) % N, N>::v;
+          t[r] = cmul(y[r][k1], cmake<T>((T)w.c, (T)(sign > 0 ? w.s : -w.s)));
+        }
+      });
+      dft<T, R>(t, sign);
+#pragma unroll
+      for (int k2 = 0; k2 < R; k2++) v[k1 + M * k2] = t[k2];
+    });
+  }
+}
+
+// Batched y pass in register form, in place: pencil p of a job (kz fastest,
+// then x, then the component) is thread p of the job's blocks.
+template <typename T, int N>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_y_reg_b(FftGroup grp, int sign) {
+  using C = typename C2<T>::t;
+  long local;
+  const FftJob* J = grp.jobs + group_member(grp, blockIdx.x, &local);
+  const int* done = guni(J->done);
+  if (done && *done) return;
+  const FftGeom g = uni(J->g);
+  const long p = local * 256 + threadIdx.x;
+  if (p >= 5L * g.nx * g.nzh) return;  // no barrier below
+  const int kz = (int)(p % g.nzh);
+  const long plane = p / g.nzh;  // c * nx + x
+  C* base = guni((C*)J->spec) + plane * (long)g.ny * g.nzp + kz;
+  C v[N];
+#pragma unroll
+  for (int y = 0; y < N; y++) v[y] = base[(long)y * g.nzp];
+  reg_fft<T, N>(v, sign);
+#pragma unroll
+  for (int y = 0; y < N; y++) base[(long)y * g.nzp] = v[y];
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
 bool make_axis_plan(int n, AxisPlan* ap) {
@@ -1288,9 +1426,41 @@ int pass_candidates(FftPass pass, int* out, int cap) {
   return k;
 }
 
+// the radices reg_fft applies for length n, first stage first (host side of
+// reg_radix: the outermost split runs last)
+int reg_form_radices(int n, int* out) {
+  int chain[12], k = 0;
+  while (n > 1 && k < 12) {
+    const int r = n == 16 ? 4 : (n % 8 == 0 && n > 8) ? 8 : (n % 4 == 0 && n > 4) ? 4 : (n % 2 == 0 && n > 2) ? 2
+                  : (n % 3 == 0 && n > 3) ? 3 : (n % 5 == 0 && n > 5) ? 5 : (n % 7 == 0 && n > 7) ? 7 : n;
+    chain[k++] = r;
+    n /= r;
+  }
+  for (int i = 0; i < k; i++) out[i] = chain[k - 1 - i];
+  return k;
+}
+
+bool reg_form_supported(int prec, FftPass pass, int n) {
+  if (pass != kPassY) return false;
+  switch (n) {  // at two waves per SIMD without spilling (FP32 64 points: 116 B)
+    case 16: case 24: case 32: return true;
+    case 48: case 64: return prec != 0;
+    default: return false;
+  }
+}
+
 PassShape pass_shape(int prec, FftPass pass, const FftGeom& g, int B) {
   const size_t csz = prec == 0 ? 16 : 8;
   PassShape ps{0, 1, 0, 0, false};
+  if (B == kRegForm || B < 0) {  // one thread per pencil, B = -(axis length)
+    const int n = pass == kPassZ ? g.nz : (pass == kPassY ? g.ny : g.nx);
+    if (!reg_form_supported(prec, pass, n) || (B < 0 && B != kRegForm && B != -n)) return ps;
+    ps.B = -n;
+    ps.blocks = (5L * g.nx * g.nzh + 255) / 256;
+    ps.lds = 0;
+    ps.ok = true;
+    return ps;
+  }
   if (pass == kPassZ) {
     ps.B = B > 0 ? B : pick_rows(g.nz, prec == 0 ? 768 : 1536);
     ps.blocks = (g.nrows + ps.B - 1) / ps.B;
@@ -1343,6 +1513,17 @@ static hipError_t z_c2r_group_t(int B, const FftGroup& grp, long blocks, size_t 
 template <typename T>
 static hipError_t y_group_t(int B, const FftGroup& grp, long blocks, size_t lds, int sign, hipStream_t st) {
   switch (B) {
+#define RCASE(NN) case -NN: hipLaunchKernelGGL((k_y_reg_b<T, NN>), dim3((unsigned)blocks), dim3(256), 0, st, grp, sign); break;
+    RCASE(16) RCASE(24) RCASE(32)
+#undef RCASE
+    case -48: case -64:
+      if constexpr (sizeof(T) == 4) {
+        if (B == -48) hipLaunchKernelGGL((k_y_reg_b<T, 48>), dim3((unsigned)blocks), dim3(256), 0, st, grp, sign);
+        else hipLaunchKernelGGL((k_y_reg_b<T, 64>), dim3((unsigned)blocks), dim3(256), 0, st, grp, sign);
+        break;
+      } else {
+        return hipErrorInvalidValue;
+      }
 #define CASE(BB) case BB: hipLaunchKernelGGL((k_y_b<T, BB>), dim3((unsigned)blocks), dim3(256), lds, st, grp, sign); break;
     CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
 #undef CASE
