@@ -31,15 +31,27 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
-# Algorithmic work per unit: the FP64 arithmetic of the Newton kernel
-# (exponent 20, uniform CRSS) counted from its gfx950 assembly, an FMA as 2
-# flops, a division or square root as 1, compares and maxima not counted.
-# Outside the Newton loop: the crystal-frame maps Q(R) and Q(R)^-1, the
-# congruence P^T L0^-1 P, the rescale, the constitutive law, the lambda update
-# and the block sums.  Per Newton step: 12 resolved shears, the power law, the
-# crystal-frame Jacobian and residual, the 5x5 LDL^T solve and the map back.
-NR_FLOPS_FIXED = 1850.0       # per voxel per iteration, outside the Newton steps
-NR_FLOPS_PER_STEP = 570.0     # per Newton step
+# Algorithmic work per unit (the per-unit figures of the measurement plan):
+# the reference's constrained Newton solve (Solvers.cpp:111-222) is taken as
+# 1.5 kflop per Newton step (12 resolved shears, the power law, the 5x5
+# Jacobian sum and J' = -I - L0 J, the residual, the 5x5 LU solve) plus
+# 0.6 kflop per voxel-iteration outside the Newton steps (the rescale, the
+# constitutive law, phi and lambda, the norms); a 1-D transform of a real
+# field costs 2.5 log2(n) flop per point, so the forward and inverse
+# transforms of the 5 components along one axis cost 25 log2(n) per voxel.
+# The Newton steps are counted by the kernel's own histogram.  The flops the
+# kernel actually issues (its crystal-frame form needs fewer per step and
+# more outside them) are read from the FP64 counter profile below.
+NR_ALG_FLOPS_FIXED = 600.0      # per voxel-iteration, outside the Newton steps
+NR_ALG_FLOPS_PER_STEP = 1500.0  # per Newton step
+
+
+def fft_axis_flops(n):
+    """Algorithmic flops per voxel of the forward + inverse transforms of the
+    five components along an axis of n points."""
+    return 25.0 * float(np.log2(n))
+
+
 # d~, sigma, lambda (40 B each), R (72 B) and one CRSS (8 B) read; sigma,
 # lambda, phi written (40 B each): the uniform-CRSS variant the workload runs
 # (every voxel's CRSS equal and kept equal by the hardening law)
@@ -50,6 +62,9 @@ NR_BYTES = 320.0
 # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a
 # plain copy of known size) for the default workload
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "strain_step_256cubed_hbm_traffic.md")
+# FP64 flops the Newton kernel issues per launch (rocprofv3 SQ_INSTS_VALU_*_F64)
+# for the default workload
+FP64_PROFILE = os.path.join(ROOT, "profiles", "nr_update_256cubed_fp64_counters.md")
 
 
 def log(*a):
@@ -63,14 +78,19 @@ def fft_bytes(kernel, n, nc, elt):
     return 2 * 5 * nc * 2 * elt  # y / x passes: read + write the half spectrum
 
 
-def measured_traffic(args, kernel):
-    """HBM bytes per launch of `kernel` from the committed counter profile, when
+def profile_entry(args, path, kernel):
+    """`kernel`'s entry of the JSON block of a committed counter profile, when
     the run is the workload that profile measured (else None)."""
-    if (args.size, args.grains, args.nexp, args.seed) != (256, 4096, 20, 0) or not os.path.exists(TRAFFIC_PROFILE):
+    if (args.size, args.grains, args.nexp, args.seed) != (256, 4096, 20, 0) or not os.path.exists(path):
         return None
     import re
-    m = re.search(r"```json\n(.*?)\n```", open(TRAFFIC_PROFILE).read(), re.S)
-    e = json.loads(m.group(1)).get(kernel) if m else None
+    m = re.search(r"```json\n(.*?)\n```", open(path).read(), re.S)
+    return json.loads(m.group(1)).get(kernel) if m else None
+
+
+def measured_traffic(args, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter profile."""
+    e = profile_entry(args, TRAFFIC_PROFILE, kernel)
     return None if e is None else e["read_bytes"] + e["write_bytes"]
 
 
@@ -225,20 +245,23 @@ def batched_workload(V, args):
                         for g, c in zip(gs, cases)})
         for g in gs:
             g.close()
-    gs = make()
-    t0 = time.perf_counter()
-    for _ in range(args.batch_steps):
-        for g, ed in zip(gs, edots):
-            g.RunSingleStrainStep(ed, args.max_iter, args.dt, args.batch_tol)
-    wall_seq = time.perf_counter() - t0
-    for g in gs:
-        g.close()
     rve_steps = args.batch_rves * args.batch_steps
+    wall_seq = None
+    if not getattr(args, "no_one_by_one", False):
+        gs = make()
+        t0 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            for g, ed in zip(gs, edots):
+                g.RunSingleStrainStep(ed, args.max_iter, args.dt, args.batch_tol)
+        wall_seq = time.perf_counter() - t0
+        for g in gs:
+            g.close()
     return {
         "rves": args.batch_rves, "voxels": voxels, "steps": args.batch_steps, "tol": args.batch_tol,
         "precision": "fp32 FFT + fp64 Newton",
         "rve_steps_per_s": rve_steps / wall, "voxel_iters_per_s": iters / wall, "wall_s": wall,
-        "one_by_one_rve_steps_per_s": rve_steps / wall_seq, "speedup_vs_one_by_one": wall_seq / wall,
+        "one_by_one_rve_steps_per_s": rve_steps / wall_seq if wall_seq else None,
+        "speedup_vs_one_by_one": wall_seq / wall if wall_seq else None,
         "tuned_block_shapes_zyx": {str(e): list(sh) for e, sh in shapes},
         "tuned_radix_plans_zyx": {str(e): list(r) for e, r in radix},
     }
@@ -296,6 +319,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-max-iters", type=int, default=20)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched workload and print its JSON (for a kernel trace of it)")
+    ap.add_argument("--no-one-by-one", action="store_true",
+                    help="batched workload without the grid-by-grid comparison run")
     ap.add_argument("--batch-rves", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-tol", type=float, default=1e-5)
@@ -327,6 +354,11 @@ def main():
     import vpfft_amd as V
     V.load_library()
 
+    if args.batched_only:
+        log(f"[bench] batched: {args.batch_rves} uneven RVEs")
+        print(json.dumps(batched_workload(V, args)), flush=True)
+        return
+
     log(f"[bench] generating {args.size}^3 microstructure, {args.grains} grains")
     R, _, _ = V.generate_voronoi(args.size, args.size, args.size, args.grains, args.seed)
     rng = np.random.default_rng(args.seed)
@@ -356,11 +388,17 @@ def main():
             entry["GBps"] = b / (ms / cnt) / 1e6
         if k in ("nr_update", "z_nr_fused"):
             # per launch (one fixed-point iteration over the grid): the fixed
-            # work of every voxel plus the Newton steps of an average iteration
-            # (the fused kernel's z transforms are not counted)
-            flops = n * NR_FLOPS_FIXED + nr_steps_total / f64["iters"] * NR_FLOPS_PER_STEP
+            # work of every voxel, the Newton steps of an average iteration
+            # and, in the fused kernel, the z transforms
+            flops = n * NR_ALG_FLOPS_FIXED + nr_steps_total / f64["iters"] * NR_ALG_FLOPS_PER_STEP
+            if k == "z_nr_fused":
+                flops += n * fft_axis_flops(args.size)
             entry["flops_per_launch"] = flops
             entry["TFLOPs"] = flops / (ms / cnt / 1e3) / 1e12
+            ex = profile_entry(args, FP64_PROFILE, k)
+            if ex is not None:
+                entry["issued_fp64_flops_per_launch"] = ex["fp64_flops_issued"]
+                entry["issued_TFLOPs"] = ex["fp64_flops_issued"] / (ms / cnt / 1e3) / 1e12
             b = NR_BYTES * n if k == "nr_update" else (NR_BYTES - 80.0) * n + 2 * 5 * nc * 16
             entry["GBps"] = b / (ms / cnt) / 1e6
         per_kernel[k] = entry
@@ -368,8 +406,13 @@ def main():
         ach = per_kernel[dom]["TFLOPs"]
         roof = {"bound": "fp64_valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach / FP64_PEAK_TFLOPS, "traffic": measured_traffic(args, dom), "kernel": dom,
-                "note": "FP64 vector ALU work (no MFMA): MI355X's FP64 vector and matrix peaks are both "
+                "note": "algorithmic flops: the reference's Newton solve at 1.5 kflop per Newton step (the "
+                        "kernel's own step histogram) + 0.6 kflop per voxel-iteration, + the z transforms; "
+                        "FP64 vector ALU work (no MFMA): MI355X's FP64 vector and matrix peaks are both "
                         "78.6 TFLOP/s"}
+        if "issued_TFLOPs" in per_kernel[dom]:
+            roof["issued_fp64_tflops"] = per_kernel[dom]["issued_TFLOPs"]
+            roof["issued_frac"] = per_kernel[dom]["issued_TFLOPs"] / FP64_PEAK_TFLOPS
     else:
         ach = per_kernel[dom]["GBps"]
         roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -407,7 +450,7 @@ def main():
         log(f"[bench] batched: {args.batch_rves} uneven RVEs")
         batched = batched_workload(V, args)
         log(f"[bench] batched: {batched['rve_steps_per_s']:.1f} RVE-steps/s, "
-            f"{batched['speedup_vs_one_by_one']:.2f}x the one-by-one rate")
+            f"{batched['speedup_vs_one_by_one'] or 0:.2f}x the one-by-one rate")
 
     serving = None
     if not args.no_serving:
