@@ -110,6 +110,39 @@ def test_batch_tuned_radix_orders_agree_to_rounding(gpu, vpfft):
             assert rel_err(b[k], a[k]) < bar, k
 
 
+def test_register_form_y_pass_against_the_single_grid_path(gpu, vpfft):
+    """The batched y pass of 16- to 64-point axes runs in register form (one
+    thread per pencil, the pencil in registers); the single-grid path runs the
+    LDS pass.  One step of a batch against the single-grid steps of the same
+    grids: FP64 to 1e-10, mixed <sigma> to 1e-6 and fields to 1e-4 (FP32
+    transforms round at ~1e-7), same iteration counts."""
+    cases = [((12, 48, 20), 9, 10, 1), ((10, 64, 16), 12, 20, 1), ((8, 32, 12), 6, 20, 0), ((10, 24, 8), 5, 10, 0),
+             ((6, 16, 10), 4, 5, 0)]
+    print("register-form y pass: batch vs single-grid steps", flush=True)
+    edots = np.array([random_unit_edot(300 + i) for i in range(len(cases))])
+    single = []
+    for i, c in enumerate(cases):
+        g = _grids(vpfft, [c], seeds=[i], seed0=90)[0]
+        a = g.RunSingleStrainStep(edots[i], 30, 0.005, 1e-6)
+        single.append((a, g.last_info.iterations, _state(g)))
+        g.close()
+    grids = _grids(vpfft, cases, seeds=list(range(len(cases))), seed0=90)
+    with vpfft.Batch() as b:
+        avg = b.RunSingleStrainStep(grids, edots, 30, 0.005, 1e-6)
+        iters = [inf.iterations for inf in b.last_infos]
+        shapes = [b.block_shape(g, 1) for g in grids]
+    assert shapes == [-c[0][1] for c in cases], shapes
+    for j, c in enumerate(cases):
+        mixed = c[3] == 1
+        assert iters[j] == single[j][1], j
+        assert rel_err(avg[j], single[j][0]) < (1e-6 if mixed else 1e-10), j
+        st = _state(grids[j])
+        for k in st:
+            assert rel_err(st[k], single[j][2][k]) < (1e-4 if mixed else 1e-10), (j, k)
+    for g in grids:
+        g.close()
+
+
 def test_batch_of_one_matches_single_grid_path(gpu, vpfft):
     print("batch of one vs vpfft_run_strain_step", flush=True)
     dims = (24, 24, 24)
