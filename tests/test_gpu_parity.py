@@ -432,17 +432,18 @@ def _fused_run(vpfft, dims, R, mat, edot, prec, fuse, graph_iters=None, profile=
     return out, fields, kt
 
 
-@pytest.mark.parametrize("prec,graph_iters,profile", [(0, None, False), (1, None, False), (0, 3, False),
-                                                      (0, None, True)])
-def test_fused_z_loop_matches_unfused(gpu, vpfft, prec, graph_iters, profile):
-    """Nz = 256: the fused inverse-z / Newton / forward-z kernel gives the bits
-    of the three separate kernels -- every step's <sigma> and
+@pytest.mark.parametrize("dims,prec,graph_iters,profile", [((8, 10, 256), 0, None, False), ((8, 10, 256), 1, None, False),
+                                                           ((8, 10, 256), 0, 3, False), ((8, 10, 256), 0, None, True),
+                                                           ((6, 8, 512), 0, None, False), ((6, 8, 512), 1, None, False),
+                                                           ((6, 8, 512), 0, 3, False), ((6, 8, 512), 1, None, True)])
+def test_fused_z_loop_matches_unfused(gpu, vpfft, dims, prec, graph_iters, profile):
+    """Nz = 256 and 512: the fused inverse-z / Newton / forward-z kernel gives
+    the bits of the three separate kernels -- every step's <sigma> and
     iteration count, and the final stress, Lagrange multiplier, polarization
     and strain-rate variation (both rebuilt after the loop), orientation and
     CRSS -- graph-replayed (graph length 4, and 3, which a fused loop rounds
     up to an even length) and host-launched."""
     from helpers import polycrystal
-    dims = (8, 10, 256)
     print("fused z loop", dims, "prec", prec, "graph", graph_iters, "profile", profile, flush=True)
     R, _ = polycrystal(0, 24, 5, dims=dims)
     mat = vpfft.Material.uniform(1.0, 20)

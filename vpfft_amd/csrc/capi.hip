@@ -107,7 +107,7 @@ struct Grid {
   hipStream_t st = nullptr;
   double *R = nullptr, *crss = nullptr, *acc = nullptr, *sigma = nullptr, *lambda = nullptr;
   void *phi = nullptr, *dtil = nullptr, *spec = nullptr;  // sized for FP64
-  // Nz = 256: the loop runs the fused inverse-z / Newton / forward-z kernel,
+  // Nz = 256 or 512: the loop runs the fused inverse-z / Newton / forward-z kernel,
   // alternating the half spectrum between spec and spec2 (VPFFT_FUSE_Z=0
   // turns it off; the separate kernels then take the fused kernel's runs of
   // 64 voxels, so both forms give the same bits)
@@ -408,7 +408,7 @@ static int check_material(const vpfft_material* m) {
 static size_t grid_bytes(const vpfft_grid_desc* d) {
   const long n = (long)d->nx * d->ny * d->nz;
   const long nc = (long)d->nx * d->ny * padded_half(d->nz);
-  const bool fused = d->nz == 256;
+  const bool fused = d->nz == 256 || d->nz == 512;
   size_t b = sizeof(double) * 42 * (size_t)n + sizeof(double2) * 5 * (size_t)nc * (fused ? 2 : 1);
   b += sizeof(StepConst) + sizeof(GreenConst<double>) + sizeof(GreenConst<float>) + sizeof(PrologueRecord) +
        sizeof(HealthRecord) + sizeof(LoopState) + sizeof(IterRecord) * 128;
@@ -508,7 +508,7 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
   ALLOC(g->phi, sizeof(double) * 5 * n);
   ALLOC(g->dtil, sizeof(double) * 5 * n);
   ALLOC(g->spec, sizeof(double2) * 5 * g->nc);
-  if (g->nz == 256) {
+  if (g->nz == 256 || g->nz == 512) {
     ALLOC(g->spec2, sizeof(double2) * 5 * g->nc);
     const char* f = std::getenv("VPFFT_FUSE_Z");
     g->fuse_z = !(f && std::atoi(f) == 0);
@@ -579,7 +579,7 @@ static int enqueue_iteration(Grid* g, int prec, double tol, int max_iter, int k)
     if ((rc = run_fft_green(g, prec, done))) return rc;
     kbegin(g, K_NR);
     HIP_TRY(launch_nr(prec, g->nexp_uniform, g->uniform_slips(), g->n, g->dtil, g->phi, g->sigma, g->lambda, g->R,
-                      g->crss, g->dC, g->partial, g->dls, g->dhealth, g->nz == 256, g->st));
+                      g->crss, g->dC, g->partial, g->dls, g->dhealth, g->nz == 256 || g->nz == 512, g->st));
     kend(g, K_NR);
   }
   g->phi_prec = prec;

@@ -486,52 +486,9 @@ __global__ void __launch_bounds__(512) k_y512(typename C2<T>::t* __restrict__ s,
   }
 }
 
-// z passes of length 512 in the y512 form (16 x 16 x 2).  A pencil is one
-// (row, pair) of real components packed as f[2 pair] + i f[2 pair + 1];
-// 16 pencils per block, 32 threads per pencil, thread t the fastest index
-// (256-byte runs along z in FP64).  pb: this pencil's 512 reals of LDS, the
-// 16 x 32 gather array with row k rotated by 2 k so the gathering reads of
-// one i spread over the banks.  On entry thread t holds
-// x[t + 32 j]; on return thread u = t holds X[k1 + 16 (m + 16 t0)] in v[m],
-// k1 = u >> 1, t0 = u & 1.  Ends with a barrier.
-template <typename T>
-__device__ __forceinline__ void fft512_z(typename C2<T>::t v[16], T* pb, const typename C2<T>::t* __restrict__ tw,
-                                         int t, int sign) {
-  using C = typename C2<T>::t;
-  dft16<T>(v, sign);
-#pragma unroll
-  for (int k = 1; k < 16; k++) {  // W512^(t k1)
-    C w = tw[t * k];
-    if (sign > 0) w.y = -w.y;
-    v[k] = cmul(v[k], w);
-  }
-  const int k1 = t >> 1, t0 = t & 1;
-  T re[16];
-#pragma unroll
-  for (int k = 0; k < 16; k++) pb[k * 32 + ((t + 2 * k) & 31)] = v[k].x;
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < 16; i++) re[i] = pb[k1 * 32 + ((2 * i + t0 + 2 * k1) & 31)];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < 16; k++) pb[k * 32 + ((t + 2 * k) & 31)] = v[k].y;
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < 16; i++) v[i] = cmake<T>(re[i], pb[k1 * 32 + ((2 * i + t0 + 2 * k1) & 31)]);
-  __syncthreads();
-  dft16<T>(v, sign);  // A_t0[m]
-#pragma unroll
-  for (int m = 0; m < 16; m++) {  // the partner (t ^ 1) is the neighbouring lane
-    C o;
-    o.x = __shfl_xor(v[m].x, 1);
-    o.y = __shfl_xor(v[m].y, 1);
-    C w = tw[16 * m];  // W32^m
-    if (sign > 0) w.y = -w.y;
-    const C a0 = t0 == 0 ? v[m] : o, a1 = cmul(t0 == 0 ? o : v[m], w);
-    v[m] = t0 == 0 ? cadd(a0, a1) : csub(a0, a1);
-  }
-}
-
+// z passes of length 512 in the y512 form (16 x 16 x 2): 16 pencils per
+// block, 32 threads per pencil, thread t the fastest index (256-byte runs
+// along z in FP64).  The transforms are z_r2c512_pencil / z_c2r512_pencil.
 template <typename T>
 __global__ void __launch_bounds__(512) k_z_r2c512(const T* __restrict__ f, typename C2<T>::t* __restrict__ s,
                                                   FftGeom g, const typename C2<T>::t* __restrict__ tw,
@@ -553,39 +510,9 @@ __global__ void __launch_bounds__(512) k_z_r2c512(const T* __restrict__ f, typen
     const T b = (active && pair < 2) ? fb[t + 32 * j] : (T)0;
     v[j] = cmake<T>(a, b);
   }
-  T* pb = buf + p * 512;
-  fft512_z<T>(v, pb, tw, t, -1);
-  // Hermitian unpack: X_a = (Z_k + conj Z_-k) / 2, X_b = -i/2 (Z_k - conj Z_-k)
-  // for k = t + 32 j <= 256; Z in natural order through LDS, real parts first
-  const int k1 = t >> 1, t0 = t & 1;
-  T zr[9], zi[9], nr[9], ni[9];
-#pragma unroll
-  for (int ph = 0; ph < 2; ph++) {
-#pragma unroll
-    for (int m = 0; m < 16; m++) pb[k1 + 16 * m + 256 * t0] = ph == 0 ? v[m].x : v[m].y;
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < 9; j++) {
-      const int k = t + 32 * j;
-      const int kn = (512 - k) & 511;
-      const T a = k <= 256 ? pb[k] : (T)0;
-      const T b = k <= 256 ? pb[kn] : (T)0;
-      if (ph == 0) { zr[j] = a; nr[j] = b; } else { zi[j] = a; ni[j] = b; }
-    }
-    __syncthreads();
-  }
-  if (active) {
-    C* sa = s + (long)(2 * pair) * g.nc + row * g.nzp;
-    C* sb = s + (long)(2 * pair + 1) * g.nc + row * g.nzp;
-    const T h = (T)0.5;
-#pragma unroll
-    for (int j = 0; j < 9; j++) {
-      const int k = t + 32 * j;
-      if (k > 256) continue;
-      sa[k] = cmake<T>(h * (zr[j] + nr[j]), h * (zi[j] - ni[j]));
-      if (pair < 2) sb[k] = cmake<T>(h * (zi[j] + ni[j]), -h * (zr[j] - nr[j]));
-    }
-  }
+  C* sa = s + (long)(2 * pair) * g.nc + row * g.nzp;
+  C* sb = pair < 2 ? s + (long)(2 * pair + 1) * g.nc + row * g.nzp : nullptr;
+  z_r2c512_pencil<T>(v, active, tw, buf + p * 512, t, sa, sb);
 }
 
 template <typename T>
@@ -601,25 +528,9 @@ __global__ void __launch_bounds__(512) k_z_c2r512(const typename C2<T>::t* __res
   const int pair = (int)(P - row * 3);
   const bool active = row < g.nrows;
   const C* sa = s + (long)(2 * pair) * g.nc + row * g.nzp;
-  const C* sb = s + (long)(2 * pair + 1) * g.nc + row * g.nzp;
+  const C* sb = pair < 2 ? s + (long)(2 * pair + 1) * g.nc + row * g.nzp : nullptr;
   C v[16];
-#pragma unroll
-  for (int j = 0; j < 16; j++) {
-    const int k = t + 32 * j;
-    const int kk = k <= 256 ? k : 512 - k;  // Z_-k from the conjugates
-    C xa = cmake<T>(0, 0), xb = cmake<T>(0, 0);
-    if (active) {
-      xa = sa[kk];
-      if (pair < 2) xb = sb[kk];
-    }
-    if (kk == 0 || kk == 256) {  // C2R reads only the real part of the self-conjugate bins
-      xa.y = 0;
-      xb.y = 0;
-    }
-    // Z_k = X_a + i X_b ;  Z_-k = conj X_a + i conj X_b
-    v[j] = k <= 256 ? cmake<T>(xa.x - xb.y, xa.y + xb.x) : cmake<T>(xa.x + xb.y, xb.x - xa.y);
-  }
-  fft512_z<T>(v, buf + p * 512, tw, t, +1);  // thread t holds z = k1 + 16 (m + 16 t0)
+  z_c2r512_pencil<T>(sa, sb, active, tw, buf + p * 512, t, v);  // thread t holds z = k1 + 16 (m + 16 t0)
   if (active) {
     const int k1 = t >> 1, t0 = t & 1;
     T* oa = f + (long)(2 * pair) * g.n + row * 512;

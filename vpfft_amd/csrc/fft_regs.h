@@ -1,5 +1,5 @@
-// fft_regs.h -- the complex helpers and the 256-point register transforms of
-// the z passes, shared by fft_green.hip (the z passes) and voxel_kernels.hip
+// fft_regs.h -- the complex helpers and the 256- and 512-point register
+// transforms of the z passes, shared by fft_green.hip (the z passes) and voxel_kernels.hip
 // (the fused inverse-z / Newton / forward-z kernel), so both compile the same
 // operations.  Every function body turns multiply-add contraction off: the
 // fused multiply-adds are written out (fmad), so a transform's bits do not
@@ -207,6 +207,129 @@ __device__ __forceinline__ void z_r2c256_pencil(typename C2<T>::t v[16], bool ac
     for (int j = 0; j < 9; j++) {
       const int k = t + 16 * j;
       if (k > 128) continue;
+      sa[k] = cmake<T>(h * (zr[j] + nr[j]), h * (zi[j] - ni[j]));
+      if (sb) sb[k] = cmake<T>(h * (zi[j] + ni[j]), -h * (zr[j] - nr[j]));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 512 = 16 x 16 x 2: thirty-two threads own a pencil.
+//  1. thread t holds x[t + 32 j] (j < 16); a 16-point DFT over j and the
+//     twiddle W512^(t k1) give Y_t[k1];
+//  2. through LDS, thread u = 2 k1 + t0 gathers Y_{2 i + t0}[k1] (i < 16) and
+//     takes a 16-point DFT over i: A_t0[m];
+//  3. X[k1 + 16 m] = A_0[m] + W32^m A_1[m], X[k1 + 16 (m + 16)] = A_0[m] -
+//     W32^m A_1[m], the partner thread (u ^ 1) the neighbouring lane: thread
+//     u ends with X[k1 + 16 (m + 16 t0)] for m < 16.
+// pb: this pencil's 512 reals of LDS, the 16 x 32 gather array with row k
+// rotated by 2 k so the gathering reads of one i spread over the banks.
+// Every lane of the block takes part (the barriers); ends with a barrier.
+// tw: W512^j, j = 0..511 (forward sign).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void fft512_z(typename C2<T>::t v[16], T* pb, const typename C2<T>::t* __restrict__ tw,
+                                         int t, int sign) {
+#pragma clang fp contract(off)
+  using C = typename C2<T>::t;
+  dft16<T>(v, sign);
+#pragma unroll
+  for (int k = 1; k < 16; k++) {  // W512^(t k1)
+    C w = tw[t * k];
+    if (sign > 0) w.y = -w.y;
+    v[k] = cmul(v[k], w);
+  }
+  const int k1 = t >> 1, t0 = t & 1;
+  T re[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) pb[k * 32 + ((t + 2 * k) & 31)] = v[k].x;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 16; i++) re[i] = pb[k1 * 32 + ((2 * i + t0 + 2 * k1) & 31)];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; k++) pb[k * 32 + ((t + 2 * k) & 31)] = v[k].y;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 16; i++) v[i] = cmake<T>(re[i], pb[k1 * 32 + ((2 * i + t0 + 2 * k1) & 31)]);
+  __syncthreads();
+  dft16<T>(v, sign);  // A_t0[m]
+#pragma unroll
+  for (int m = 0; m < 16; m++) {  // the partner (t ^ 1) is the neighbouring lane
+    C o;
+    o.x = __shfl_xor(v[m].x, 1);
+    o.y = __shfl_xor(v[m].y, 1);
+    C w = tw[16 * m];  // W32^m
+    if (sign > 0) w.y = -w.y;
+    const C a0 = t0 == 0 ? v[m] : o, a1 = cmul(t0 == 0 ? o : v[m], w);
+    v[m] = t0 == 0 ? cadd(a0, a1) : csub(a0, a1);
+  }
+}
+
+// Inverse z transform of one pencil of length 512 (32 threads, thread t of
+// the pencil): the half spectra sa, sb (bins 0..256) of real components a
+// and b packed as a + i b (sb null: b is 0).  On return thread t holds a + i b
+// at z = k1 + 16 (m + 16 t0) in v[m] (k1 = t >> 1, t0 = t & 1), unscaled.
+template <typename T>
+__device__ __forceinline__ void z_c2r512_pencil(const typename C2<T>::t* __restrict__ sa,
+                                                const typename C2<T>::t* __restrict__ sb, bool active,
+                                                const typename C2<T>::t* __restrict__ tw, T* pb, int t,
+                                                typename C2<T>::t v[16]) {
+#pragma clang fp contract(off)
+  using C = typename C2<T>::t;
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const int k = t + 32 * j;
+    const int kk = k <= 256 ? k : 512 - k;  // Z_-k from the conjugates
+    C xa = cmake<T>(0, 0), xb = cmake<T>(0, 0);
+    if (active) {
+      xa = sa[kk];
+      if (sb) xb = sb[kk];
+    }
+    if (kk == 0 || kk == 256) {  // C2R reads only the real part of the self-conjugate bins
+      xa.y = 0;
+      xb.y = 0;
+    }
+    // Z_k = X_a + i X_b ;  Z_-k = conj X_a + i conj X_b
+    v[j] = k <= 256 ? cmake<T>(xa.x - xb.y, xa.y + xb.x) : cmake<T>(xa.x + xb.y, xb.x - xa.y);
+  }
+  fft512_z<T>(v, pb, tw, t, +1);
+}
+
+// Forward z transform of one pencil of length 512 from v (thread t holds
+// a + i b at z = t + 32 j in v[j]) to the half spectra sa, sb (bins 0..256;
+// sb null: b is not stored).  Every lane of the block takes part.
+template <typename T>
+__device__ __forceinline__ void z_r2c512_pencil(typename C2<T>::t v[16], bool active,
+                                                const typename C2<T>::t* __restrict__ tw, T* pb, int t,
+                                                typename C2<T>::t* __restrict__ sa, typename C2<T>::t* __restrict__ sb) {
+#pragma clang fp contract(off)
+  fft512_z<T>(v, pb, tw, t, -1);
+  // Hermitian unpack: X_a = (Z_k + conj Z_-k) / 2, X_b = -i/2 (Z_k - conj Z_-k)
+  // for k = t + 32 j <= 256; Z in natural order through LDS, real parts first
+  const int k1 = t >> 1, t0 = t & 1;
+  T zr[9], zi[9], nr[9], ni[9];
+#pragma unroll
+  for (int ph = 0; ph < 2; ph++) {
+#pragma unroll
+    for (int m = 0; m < 16; m++) pb[k1 + 16 * m + 256 * t0] = ph == 0 ? v[m].x : v[m].y;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+      const int k = t + 32 * j;
+      const int kn = (512 - k) & 511;
+      const T a = k <= 256 ? pb[k] : (T)0;
+      const T b = k <= 256 ? pb[kn] : (T)0;
+      if (ph == 0) { zr[j] = a; nr[j] = b; } else { zi[j] = a; ni[j] = b; }
+    }
+    __syncthreads();
+  }
+  if (active) {
+    const T h = (T)0.5;
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+      const int k = t + 32 * j;
+      if (k > 256) continue;
       sa[k] = cmake<T>(h * (zr[j] + nr[j]), h * (zi[j] - ni[j]));
       if (sb) sb[k] = cmake<T>(h * (zi[j] + ni[j]), -h * (zr[j] - nr[j]));
     }

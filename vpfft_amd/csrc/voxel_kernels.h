@@ -123,7 +123,8 @@ hipError_t launch_taylor_batch(long n, const double* R, const double* crss, cons
                                double* s_out, hipStream_t st);
 
 // The fused inverse-z / per-voxel body / forward-z kernel of one iteration
-// (Nz = 256), and the rebuild of phi = sigma - L0 d~ after a fused loop.  The fused kernel reads the half spectrum spec_in and writes
+// (Nz = 256 or 512), and the rebuild of phi = sigma - L0 d~ after a fused
+// loop.  The fused kernel reads the half spectrum spec_in and writes
 // spec_out (different buffers), fills the same partial rows as launch_nr and
 // folds them the same way.
 bool zfused_eligible(const FftGeom& g);

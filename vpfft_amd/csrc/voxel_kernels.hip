@@ -579,7 +579,12 @@ __device__ __forceinline__ VoxJob vox_job(const VoxJob* J) {
 // of different waves overlap on a CU.  The operations are those of
 // k_z_c2r256, the single form with zf.linear set (a block takes 64
 // consecutive voxels) and k_z_r2c256, so the bits are the same.
-enum NrMode { NR_SINGLE = 0, NR_BATCHED = 1, NR_ZFUSED = 2 };
+// NR_ZFUSED512 (Nz = 512) is the same with rows of 512 voxels (8 runs):
+// a pencil takes 32 lanes (the 16 x 16 x 2 form of k_z_c2r512 / k_z_r2c512),
+// so the wave transforms the component pairs (0, 1) and (2, 3) side by side
+// and component 4 alone, and the transforms' LDS lies inside the row's
+// 5 x 512 values (20 KB in FP64: eight waves fill a CU's 160 KB).
+enum NrMode { NR_SINGLE = 0, NR_BATCHED = 1, NR_ZFUSED = 2, NR_ZFUSED512 = 3 };
 
 struct ZFusedArgs {
   const void* spec_in;
@@ -597,7 +602,9 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
          double* __restrict__ lambda_, const double* __restrict__ R_, const double* __restrict__ crss_,
          const StepConst* __restrict__ Cp_, double* __restrict__ partial_, const LoopState* __restrict__ ls_,
          HealthRecord* __restrict__ health_, VoxGroup grp, ZFusedArgs zf) {
-  constexpr bool FUSED = MODE == NR_ZFUSED;
+  constexpr bool FUSED = MODE == NR_ZFUSED || MODE == NR_ZFUSED512;
+  constexpr int ZN = MODE == NR_ZFUSED512 ? 512 : 256;  // a fused block's row
+  constexpr int RUNS = ZN / 64;
   using Cx = typename C2<T>::t;
   long n, blk;
   const T* dtil;
@@ -638,12 +645,45 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
   }
   // the fused form's LDS: the exchange buffer of the z transforms and the
   // row's d~ (then phi), [component][z]
-  __shared__ T zbuf[FUSED ? zbuf_reals<3>() : 1];
-  __shared__ T fld[FUSED ? 5 * 256 : 1];
+  __shared__ T zbuf[MODE == NR_ZFUSED ? zbuf_reals<3>() : 1];
+  __shared__ T fld[FUSED ? 5 * ZN : 1];
   const int zt16 = threadIdx.x & 15, zpair = threadIdx.x >> 4;
   const bool zactive = zpair < 3;
   const long zrow = blockIdx.x;
-  if constexpr (FUSED) {  // d~ row: inverse z transforms into LDS
+  if constexpr (MODE == NR_ZFUSED512) {  // d~ row: inverse z transforms into LDS
+    // component 4 first, its transform's LDS in the rows of components 0 and
+    // 1 (not written yet); then the pairs (0, 1) and (2, 3), whose LDS
+    // (the same rows) the transform has left when the values are written
+    // (each transform takes its lane coordinates afresh: the compiler would
+    // otherwise keep the first one's twiddle and LDS addresses live, in
+    // scratch, for the second)
+    const Cx* si = (const Cx*)zf.spec_in;
+    const int half = threadIdx.x >> 5;
+    Cx v[16];
+    {
+      const int t = threadIdx.x & 31, k1 = t >> 1, t0 = t & 1;
+      z_c2r512_pencil<T>(si + 4L * zf.nc + zrow * zf.nzp, nullptr, half == 0, (const Cx*)zf.tw, fld + half * 512, t, v);
+      if (half == 0) {
+#pragma unroll
+        for (int m = 0; m < 16; m++) fld[4 * 512 + k1 + 16 * m + 256 * t0] = v[m].x * (T)zf.scale;
+      }
+    }
+    {
+      int t = threadIdx.x & 31;
+      __asm__ volatile("" : "+v"(t));
+      const int k1 = t >> 1, t0 = t & 1;
+      const Cx* sa = si + (long)(2 * half) * zf.nc + zrow * zf.nzp;
+      z_c2r512_pencil<T>(sa, sa + zf.nc, true, (const Cx*)zf.tw, fld + half * 512, t, v);
+#pragma unroll
+      for (int m = 0; m < 16; m++) {
+        const int z = k1 + 16 * m + 256 * t0;
+        fld[(2 * half) * 512 + z] = v[m].x * (T)zf.scale;
+        fld[(2 * half + 1) * 512 + z] = v[m].y * (T)zf.scale;
+      }
+    }
+    __syncthreads();
+  }
+  if constexpr (MODE == NR_ZFUSED) {  // d~ row: inverse z transforms into LDS
     const Cx* si = (const Cx*)zf.spec_in;
     const Cx* sa = si + (long)(2 * zpair) * zf.nc + zrow * zf.nzp;
     const Cx* sb = zpair < 2 ? si + (long)(2 * zpair + 1) * zf.nc + zrow * zf.nzp : nullptr;
@@ -659,7 +699,7 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
     __syncthreads();
   }
   const StepConst& C = *as_const_mem(Cp);
-  for (int q = 0; q < (FUSED ? 4 : 1); q++) {
+  for (int q = 0; q < (FUSED ? RUNS : 1); q++) {
     double acc[15];
 #pragma unroll
     for (int w = 0; w < 15; w++) acc[w] = 0.0;
@@ -668,8 +708,8 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
     long v, slot = 0;
     if constexpr (FUSED) {
       slot = q * 64 + (threadIdx.x & 63);
-      v = zrow * 256 + slot;
-      blk = zrow * 4 + q;
+      v = zrow * ZN + slot;
+      blk = zrow * RUNS + q;
     } else if (MODE == NR_SINGLE && zf.linear) {
       v = blk * kNrThreads + threadIdx.x;
     } else {
@@ -705,7 +745,7 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
       // q = L0^-1 lambda + edot + d~ ; lambda and d~ are re-read after the solve
       double qv[5];
 #pragma unroll
-      for (int i = 0; i < 5; i++) qv[i] = C.edot[i] + (FUSED ? (double)fld[i * 256 + slot] : ld(dtil + i * n + v));
+      for (int i = 0; i < 5; i++) qv[i] = C.edot[i] + (FUSED ? (double)fld[i * ZN + slot] : ld(dtil + i * n + v));
 #pragma unroll
       for (int j = 0; j < 5; j++) {
         const double lj = lambda[j * n + v];
@@ -743,7 +783,7 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
       double d[5], lam[5];
 #pragma unroll
       for (int i = 0; i < 5; i++) {
-        d[i] = FUSED ? (double)fld[i * 256 + slot] : ld(dtil + i * n + v);
+        d[i] = FUSED ? (double)fld[i * ZN + slot] : ld(dtil + i * n + v);
         lam[i] = lambda[i * n + v];
       }
       double Ld[5], t[5], Lt[5];
@@ -755,7 +795,7 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
       for (int i = 0; i < 5; i++) {
         lam[i] += Lt[i];
-        if constexpr (FUSED) fld[i * 256 + slot] = (T)(s[i] - Ld[i]);
+        if constexpr (FUSED) fld[i * ZN + slot] = (T)(s[i] - Ld[i]);
         else phi[i * n + v] = (T)(s[i] - Ld[i]);
         sigma[i * n + v] = s[i];
         lambda[i * n + v] = lam[i];
@@ -787,7 +827,34 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
     // lane before the reduction and 64 x that maximum after it)
     wave_partial_row(acc, mx[0], mx[1], partial + blk * kPartialStride);
   }
-  if constexpr (FUSED) {  // phi row: forward z transforms from LDS
+  if constexpr (MODE == NR_ZFUSED512) {  // phi row: forward z transforms from LDS
+    // the pairs (0, 1) and (2, 3) first, their transform's LDS in the rows of
+    // components 0 and 1 once every lane has read its values; then
+    // component 4 from its own row
+    __syncthreads();
+    const int half = threadIdx.x >> 5;
+    Cx* so = (Cx*)zf.spec_out;
+    Cx v[16];
+    {
+      int t = threadIdx.x & 31;
+      __asm__ volatile("" : "+v"(t));
+#pragma unroll
+      for (int j = 0; j < 16; j++)
+        v[j] = cmake<T>(fld[(2 * half) * 512 + t + 32 * j], fld[(2 * half + 1) * 512 + t + 32 * j]);
+      __syncthreads();
+      Cx* sa = so + (long)(2 * half) * zf.nc + zrow * zf.nzp;
+      z_r2c512_pencil<T>(v, true, (const Cx*)zf.tw, fld + half * 512, t, sa, sa + zf.nc);
+    }
+    {
+      int t = threadIdx.x & 31;
+      __asm__ volatile("" : "+v"(t));
+#pragma unroll
+      for (int j = 0; j < 16; j++) v[j] = cmake<T>(half == 0 ? fld[4 * 512 + t + 32 * j] : (T)0, (T)0);
+      z_r2c512_pencil<T>(v, half == 0, (const Cx*)zf.tw, fld + half * 512, t, so + 4L * zf.nc + zrow * zf.nzp,
+                         nullptr);
+    }
+  }
+  if constexpr (MODE == NR_ZFUSED) {  // phi row: forward z transforms from LDS
     __syncthreads();
     // the pencil coordinates afresh: the compiler would otherwise keep the
     // inverse transforms' twiddle addresses (the same tw[t k]) live through
@@ -1329,7 +1396,7 @@ hipError_t launch_nr(int prec, int nexp_uniform, bool uniform_tau, long n, const
   return hipGetLastError();
 }
 
-bool zfused_eligible(const FftGeom& g) { return g.nz == 256; }
+bool zfused_eligible(const FftGeom& g) { return g.nz == 256 || g.nz == 512; }
 
 hipError_t launch_zfused(int prec, int nexp_uniform, bool uniform_tau, long n, const void* spec_in, void* spec_out,
                          const FftGeom& g, const void* tw, double scale, double* sigma, double* lambda,
@@ -1339,8 +1406,13 @@ hipError_t launch_zfused(int prec, int nexp_uniform, bool uniform_tau, long n, c
   const unsigned blocks = (unsigned)g.nrows;
   const ZFusedArgs zf{spec_in, spec_out, tw, g.nc, g.nzp, scale, 0};
 #define ZF_LAUNCH(TT, NN, UU)                                                                                        \
-  hipLaunchKernelGGL((k_nr<TT, NN, UU, NR_ZFUSED>), dim3(blocks), dim3(kNrThreads), 0, st, n, (const TT*)nullptr,    \
-                     (TT*)nullptr, sigma, lambda, R, crss, C, partial, ls, health, VoxGroup{}, zf)
+  if (g.nz == 512)                                                                                                   \
+    hipLaunchKernelGGL((k_nr<TT, NN, UU, NR_ZFUSED512>), dim3(blocks), dim3(kNrThreads), 0, st, n,                   \
+                       (const TT*)nullptr, (TT*)nullptr, sigma, lambda, R, crss, C, partial, ls, health, VoxGroup{},  \
+                       zf);                                                                                          \
+  else                                                                                                               \
+    hipLaunchKernelGGL((k_nr<TT, NN, UU, NR_ZFUSED>), dim3(blocks), dim3(kNrThreads), 0, st, n, (const TT*)nullptr,  \
+                       (TT*)nullptr, sigma, lambda, R, crss, C, partial, ls, health, VoxGroup{}, zf)
 #define ZF_N(TT, UU)                                 \
   switch (nexp_uniform) {                             \
     case 5: ZF_LAUNCH(TT, 5, UU); break;              \
