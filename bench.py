@@ -56,7 +56,7 @@ def fft_axis_flops(n):
 # lambda, phi written (40 B each): the uniform-CRSS variant the workload runs
 # (every voxel's CRSS equal and kept equal by the hardening law)
 NR_BYTES = 320.0
-# the fused inverse-z / Newton / forward-z kernel (Nz = 256): the Newton
+# the fused inverse-z / Newton / forward-z kernel (Nz = 256 or 512): the Newton
 # kernel's bytes without d~ and phi, plus the half spectrum read and written
 # (5 components x (Nz/2 + 1) complex bins per z row, each way)
 # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a
