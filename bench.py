@@ -315,6 +315,9 @@ def main():
                     help="also run the FP64 form with the strain rate moved by this relative amount and report "
                          "its per-voxel stress change beside the mixed form's error")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="time only the oracle's fixed-point loop (the cpu_baseline leg) and print its JSON; "
+                         "no GPU is used")
     ap.add_argument("--cpu-size", type=int, default=96)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-max-iters", type=int, default=20)
@@ -352,6 +355,13 @@ def main():
         return
 
     import vpfft_amd as V
+
+    if args.cpu_only:
+        rng = np.random.default_rng(args.seed)
+        edot = rng.standard_normal(5)
+        print(json.dumps(cpu_baseline(V, args, edot / np.linalg.norm(edot))), flush=True)
+        return
+
     V.load_library()
 
     if args.batched_only:
