@@ -978,9 +978,10 @@ static int pass_axis(FftPass pass) { return pass == kPassZ ? 2 : (pass == kPassY
 // The schedule of an FFT pass for grid g (job `job` of the step's table):
 // the default one, or the fastest of the candidates when tuning is on,
 // measured once per (precision, pass, grid shape) on the batch's stream with
-// this job's own data (the batched passes are the generic Stockham ones, so
-// every axis has a radix plan).  Radix orders are tried only at tune level 2.
-// The job's plan for the pass's axis is left at the choice.
+// the data of the step's grids of that shape, all in one launch (the batched
+// passes are the generic Stockham ones, so every axis has a radix plan).
+// Radix orders are tried only at tune level 2.  The job's plan for the
+// pass's axis is left at the choice.
 // The default schedule of a batched pass: the register form where the axis
 // has one, else the generic pass's default block shape.
 static Sched default_sched(const Grid* g, FftPass pass) {
@@ -988,7 +989,11 @@ static Sched default_sched(const Grid* g, FftPass pass) {
   return Sched{rf.ok ? rf.B : pass_shape(g->prec, pass, g->geom, 0).B, 0};
 }
 
-static int choose_sched(Batch* b, Grid* g, FftPass pass, int job, Sched* out) {
+// same: the step's active jobs with g's precision and shape (job among
+// them); a schedule is timed on all of them in one launch, as the step will
+// run them, not on one grid alone (a small grid alone leaves most of the
+// GPU idle and favours the schedules with the most blocks).
+static int choose_sched(Batch* b, Grid* g, FftPass pass, int job, const std::vector<int>& same, Sched* out) {
   AxisPlan* jp = job_plan(b->hfft + job, pass);
   const int axis = pass_axis(pass);
   const int n = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
@@ -1028,21 +1033,27 @@ static int choose_sched(Batch* b, Grid* g, FftPass pass, int job, Sched* out) {
     }
   }
   float best_ms = 1e30f;
-  // a one-job group at the end of the pools (the step's launches use the front)
-  const int at = b->pool_cap - 4;
-  b->hids[at] = job;
-  b->hoff[at + 1] = 0;
+  // the group at the end of the pools (the step's launches use at most
+  // their first 14 entries per grid, from the front)
+  const int m = (int)same.size();
+  const int at = b->pool_cap - m - 2;
+  if (m < 1 || at < b->pool_cap / 2) return set_error(VPFFT_E_INVALID, "batch tuning group of %d grids", m);
+  for (int i = 0; i < m; i++) b->hids[at + i] = same[i];
   for (int order : orders) {
-    set_plan(order);
-    HIP_TRY(hipMemcpyAsync(b->dfft + job, b->hfft + job, sizeof(FftJob), hipMemcpyHostToDevice, b->st));
+    for (int j : same) {
+      AxisPlan* p = job_plan(b->hfft + j, pass);
+      if (order == 0) *p = g->plan[axis];
+      else make_axis_plan_ordered(n, kRadixOrders[order], p);
+      HIP_TRY(hipMemcpyAsync(b->dfft + j, b->hfft + j, sizeof(FftJob), hipMemcpyHostToDevice, b->st));
+    }
     for (int c = 0; c < nc; c++) {
       const PassShape ps = pass_shape(g->prec, pass, g->geom, cand[c]);
       if (!ps.ok || (ps.B < 0 && order != orders[0])) continue;  // the register form has no radix plan to vary
-      b->hoff[at + 2] = ps.blocks;
-      HIP_TRY(hipMemcpyAsync(b->dids + at, b->hids + at, sizeof(int), hipMemcpyHostToDevice, b->st));
-      HIP_TRY(hipMemcpyAsync(b->doff + at + 1, b->hoff + at + 1, 2 * sizeof(long), hipMemcpyHostToDevice, b->st));
+      for (int i = 0; i <= m; i++) b->hoff[at + i] = (long)i * ps.blocks;
+      HIP_TRY(hipMemcpyAsync(b->dids + at, b->hids + at, sizeof(int) * m, hipMemcpyHostToDevice, b->st));
+      HIP_TRY(hipMemcpyAsync(b->doff + at, b->hoff + at, sizeof(long) * (m + 1), hipMemcpyHostToDevice, b->st));
       BLaunch L{pass == kPassZ ? (int)L_ZF : (pass == kPassY ? (int)L_YF : (int)L_X), g->prec, ps.B, ps.E, 0, 0,
-                ps.blocks, ps.lds, at, at + 1, 1};
+                (long)m * ps.blocks, ps.lds, at, at, m};
       // one warm-up launch, then the best of three timings of 12 launches:
       // a small grid's pass takes a few microseconds, so fewer launches
       // would rank candidates by timer noise
@@ -1104,6 +1115,10 @@ static int batch_plan(Batch* b, Grid** gs, int count, const double* dt, const st
     nr[std::make_tuple(gs[j]->prec, gs[j]->nexp_uniform, gs[j]->uniform_slips() ? 1 : 0)].push_back(j);
     if (dt[j] > 0) epi.push_back(j);
   }
+  // the active grids of each precision and shape (a schedule is tuned on all of them)
+  std::map<std::tuple<int, int, int, int>, std::vector<int>> shapes;
+  for (int j = 0; j < count; j++)
+    if (active[j]) shapes[std::make_tuple(gs[j]->prec, gs[j]->nx, gs[j]->ny, gs[j]->nz)].push_back(j);
   // FFT passes: groups by (precision, block shape)
   const int kinds[5] = {L_ZF, L_YF, L_X, L_YI, L_ZI};
   for (int kind : kinds) {
@@ -1117,7 +1132,8 @@ static int batch_plan(Batch* b, Grid** gs, int count, const double* dt, const st
         auto it = b->tuned.find(key);
         sc = (b->tune && it != b->tuned.end()) ? it->second : default_sched(gs[j], pass);
       } else {
-        int rc = choose_sched(b, gs[j], pass, j, &sc);
+        int rc = choose_sched(b, gs[j], pass, j, shapes[std::make_tuple(gs[j]->prec, gs[j]->nx, gs[j]->ny, gs[j]->nz)],
+                              &sc);
         if (rc) return rc;
       }
       const PassShape ps = pass_shape(gs[j]->prec, pass, gs[j]->geom, sc.B);
