@@ -23,6 +23,7 @@ constexpr int kSlips = 12;
 struct StepConst {
   double L0[25];      // SMatrix5x5 column-major, symmetrised
   double L0inv[25];   // inverse of L0 (column-major, symmetric)
+  double L0inv_w[15]; // W lower triangular with L0inv = W W^T, packed by rows (W[sidx(i, j)], i >= j)
   double edot[5];     // macroscopic strain rate (EigenRep)
   double gdot0[kSlips];
   int nexp[kSlips];
@@ -389,17 +390,18 @@ __device__ __forceinline__ void rot_t(const double Q[25], const double x[5], dou
   }
 }
 
-// P^T A P for a symmetric A given column-major (the constant L0^-1); packed lower result.
-__device__ __forceinline__ void congruence_t(const double P[25], const double* __restrict__ Acm, double out[15]) {
-  double T[25];  // T = A P
+// P^T A P for A = W W^T (W lower triangular, packed by rows): M = W^T P, then
+// M^T M; packed lower result (150 multiply-adds; 200 for P^T (A P)).
+__device__ __forceinline__ void congruence_chol(const double P[25], const double* __restrict__ W, double out[15]) {
+  double M[25];
 #pragma unroll
   for (int i = 0; i < 5; i++)
 #pragma unroll
     for (int c = 0; c < 5; c++) {
       double t = 0.0;
 #pragma unroll
-      for (int j = 0; j < 5; j++) t += Acm[j * 5 + i] * P[j * 5 + c];
-      T[i * 5 + c] = t;
+      for (int j = i; j < 5; j++) t += W[sidx(j, i)] * P[j * 5 + c];
+      M[i * 5 + c] = t;
     }
 #pragma unroll
   for (int a = 0; a < 5; a++)
@@ -407,7 +409,7 @@ __device__ __forceinline__ void congruence_t(const double P[25], const double* _
     for (int c = 0; c <= a; c++) {
       double t = 0.0;
 #pragma unroll
-      for (int i = 0; i < 5; i++) t += P[i * 5 + a] * T[i * 5 + c];
+      for (int i = 0; i < 5; i++) t += M[i * 5 + a] * M[i * 5 + c];
       out[sidx(a, c)] = t;
     }
 }
@@ -686,7 +688,10 @@ __device__ __forceinline__ void apply_constitutive(const double sc[5], const dou
 #pragma unroll
   for (int k = 0; k < kSlips; k++) {
     const double r = U ? schmid_dot(k, u) : schmid_dot(k, u) * inv_tau[k];
-    const double g = U ? ipow_c<NE>(fabs(r)) * sgn(r) : gamma_n<NE>(C, k, fabs(r)) * sgn(r);
+    // the rate with the sign of r (copysign: one bit operation, where the
+    // product by sgn(r) took a compare, a select, a conversion and a multiply;
+    // only a zero rate's sign can differ, and it adds nothing)
+    const double g = __builtin_copysign(U ? ipow_c<NE>(fabs(r)) : gamma_n<NE>(C, k, fabs(r)), r);
 #pragma unroll
     for (int i = 0; i < 5; i++) {
       const int si = kSign[k][i];

@@ -386,6 +386,20 @@ __device__ bool derive_constants(StepConst* C, GreenConst<double>* g64, GreenCon
       C->L0inv[j * 5 + i] = v;
       C->L0inv[i * 5 + j] = v;
     }
+  // its Cholesky factor, for the Newton kernel's P^T L0^-1 P (a medium that is
+  // not positive definite fails the step on the host)
+  for (int j = 0; j < 5; j++) {
+    double d = C->L0inv[j * 5 + j];
+    for (int k = 0; k < j; k++) d -= C->L0inv_w[sidx(j, k)] * C->L0inv_w[sidx(j, k)];
+    ok = ok && d > 0.0;
+    const double wjj = d > 0.0 ? sqrt(d) : 0.0;
+    C->L0inv_w[sidx(j, j)] = wjj;
+    for (int i = j + 1; i < 5; i++) {
+      double t = C->L0inv[j * 5 + i];
+      for (int k = 0; k < j; k++) t -= C->L0inv_w[sidx(i, k)] * C->L0inv_w[sidx(j, k)];
+      C->L0inv_w[sidx(i, j)] = wjj > 0.0 ? t / wjj : 0.0;
+    }
+  }
   // L_ijkl = sum_ab L0(a,b) B_a,ij B_b,kl ; K_ik = sum_jl L_ijkl k_j k_l
   const double s2 = 0.70710678118654752440, s6 = 0.40824829046386301637;
   double Bm[5][3][3];
@@ -759,7 +773,7 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
         double P[25], qc[5], A0[15];
         cf::inverse_map(r9, P);
         cf::rot_t(P, qv, qc);
-        cf::congruence_t(P, C.L0inv, A0);
+        cf::congruence_chol(P, C.L0inv_w, A0);
         cf::gram(P, G);
         cf::constrained_nr<NE, UNIFORM_TAU && (NE > 0)>(sc, G, it, C, qc, A0, 5e-4, 1e9, 100, &err, &steps);
       }
@@ -1161,7 +1175,7 @@ __global__ void k_constrained_batch(long n, const double* __restrict__ R, const 
     matvec5(C.L0inv, l, Ll);
     for (int i = 0; i < 5; i++) q[i] = Ll[i] + C.edot[i] + d[i];
     cf::rot_t(P, q, qc);
-    cf::congruence_t(P, C.L0inv, A0);
+    cf::congruence_chol(P, C.L0inv_w, A0);
     double Q[25], sc[5], G[15], gs[5];
     cf::gram(P, G);
     cf::dev_map(r, Q);
