@@ -545,6 +545,11 @@ static int grid_create(const vpfft_grid_desc* d, const vpfft_material* mat, Grid
 // The fused loop applies to this step: the grid is eligible and its z pass
 // uses the 256-point register form (the default block shape).
 static bool fused_step(const Grid* g, int prec) { return g->fuse_z && g->fftB[prec][0] == 0; }
+// The Newton waves' voxels for the grids that can run the fused loop, fused
+// or not, so both forms sum the same way: a z row of 256 (one partial row per
+// wave) at Nz = 256, runs of 64 consecutive voxels at Nz = 512; else 4 x 16
+// tiles (launch_nr's row_runs)
+static int row_runs(const Grid* g) { return g->nz == 256 ? 4 : (g->nz == 512 ? 1 : 0); }
 
 // Iteration k of a fused loop: y, x + Green, y inverse on the half spectrum
 // the previous iteration left (spec for even k, spec2 for odd k), then the
@@ -579,12 +584,12 @@ static int enqueue_iteration(Grid* g, int prec, double tol, int max_iter, int k)
     if ((rc = run_fft_green(g, prec, done))) return rc;
     kbegin(g, K_NR);
     HIP_TRY(launch_nr(prec, g->nexp_uniform, g->uniform_slips(), g->n, g->dtil, g->phi, g->sigma, g->lambda, g->R,
-                      g->crss, g->dC, g->partial, g->dls, g->dhealth, g->nz == 256 || g->nz == 512, g->st));
+                      g->crss, g->dC, g->partial, g->dls, g->dhealth, row_runs(g), g->st));
     kend(g, K_NR);
   }
   g->phi_prec = prec;
   kbegin(g, K_ITER_FINALIZE);
-  HIP_TRY(launch_iter_finalize(g->n, g->partial, g->dC, tol, max_iter, g->dls, g->drec, g->st));
+  HIP_TRY(launch_iter_finalize(g->n, row_runs(g), g->partial, g->dC, tol, max_iter, g->dls, g->drec, g->st));
   kend(g, K_ITER_FINALIZE);
   return VPFFT_OK;
 }

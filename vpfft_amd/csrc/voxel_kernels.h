@@ -22,6 +22,10 @@ __host__ __device__ inline int nr_blocks(long n) { return (int)((n + kNrThreads 
 __host__ __device__ inline int pt_blocks(long n) { return (int)((n + kPointThreads - 1) / kPointThreads); }
 // partial-sum buffer rows needed by launch_nr (+ its folded rows)
 inline long nr_partial_rows(long n) { return nr_blocks(n) + (nr_blocks(n) + 255) / 256; }
+// partial rows one Newton launch writes: one per wave, a wave taking
+// row_runs runs of 64 consecutive voxels (a z row of 256) or, for
+// row_runs <= 1, one run
+inline int nr_rows(long n, int row_runs) { return row_runs > 1 ? (int)(n / (64L * row_runs)) : nr_blocks(n); }
 
 inline int point_blocks(long n) {
   long b = (n + kPointThreads - 1) / kPointThreads;
@@ -99,12 +103,14 @@ hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenCon
 // and the same reference slip rate (C.gdot0[0] is read)
 // ls: the loop state (kernels return at entry once ls->done is set; may be
 // null outside a loop); health: where the first non-finite value is recorded;
-// linear: each wave takes 64 consecutive voxels (the fused kernel's runs)
-// instead of a 4 x 16 tile
+// row_runs: 0, each wave takes a 4 x 16 tile; 1, 64 consecutive voxels (a
+// run of the Nz = 512 fused kernel); 4, a z row of 256 voxels with one
+// partial row (the Nz = 256 fused kernel's; n a multiple of 256)
 hipError_t launch_nr(int prec, int nexp_uniform, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
                      double* lambda, const double* R, const double* crss, const StepConst* C, double* partial,
-                     const LoopState* ls, HealthRecord* health, bool linear, hipStream_t st);
-hipError_t launch_iter_finalize(long n, const double* partial, const StepConst* C, double tol,
+                     const LoopState* ls, HealthRecord* health, int row_runs, hipStream_t st);
+// row_runs: as the loop's Newton launch (4 after launch_zfused at Nz = 256, 1 at Nz = 512)
+hipError_t launch_iter_finalize(long n, int row_runs, const double* partial, const StepConst* C, double tol,
                                 int max_iter, LoopState* ls, IterRecord* rec, hipStream_t st);
 hipError_t launch_loop_reset(LoopState* ls, hipStream_t st);
 hipError_t launch_epilogue(long n, double* R, double* crss, double* acc, const double* sigma, const StepConst* C,

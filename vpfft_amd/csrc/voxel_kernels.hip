@@ -155,15 +155,17 @@ __device__ __forceinline__ double wave_max_all(double x) {
 }
 
 // The per-wave row of the Newton kernel's partial sums: 14 sums of acc (slot
-// 14 of acc is not read), then 64 x the wave's largest Newton step count, the
-// largest NR error and the largest step count; lanes 0..16 write out[0..16].
+// 14 of acc is not read), then the Newton steps the wave ran for its lanes
+// (wsteps; if negative, 64 x the wave's largest step count: one run of 64
+// voxels), the largest NR error and the largest step count; lanes 0..16
+// write out[0..16].
 // The sums are a reduce-scatter in a fixed order: inside each row of 16 lanes
 // the pairs (row mirror, half-row mirror, quad reversal, quad swap) each
 // exchange half of what they hold, so a lane ends with one value summed over
 // its row, and lane swaps add the four rows.  ~120 instructions for the 14
 // sums against ~300 for one full reduction per value.
 __device__ __forceinline__ void wave_partial_row(const double (&acc)[15], double max_err, double max_steps,
-                                                 double* out) {
+                                                 double* out, double wsteps = -1.0) {
   const int lane = threadIdx.x & 63;
   double lo8[8], hi8[8], s8[8];
 #pragma unroll
@@ -185,7 +187,7 @@ __device__ __forceinline__ void wave_partial_row(const double (&acc)[15], double
   pair_halve<0xB1>(lo1, hi1, lane & 1, s1);  // quad_perm [1,0,3,2]: l, l ^ 1
   const double sum = cross_rows<true>(s1[0]);  // value lane % 16, over the wave
   const double me = wave_max_all(max_err), ms = wave_max_all(max_steps);
-  const double val = lane < 14 ? sum : lane == 14 ? 64.0 * ms : lane == 15 ? me : ms;
+  const double val = lane < 14 ? sum : lane == 14 ? (wsteps < 0.0 ? 64.0 * ms : wsteps) : lane == 15 ? me : ms;
   if (lane < 17) out[lane] = val;
 }
 
@@ -591,14 +593,22 @@ __device__ __forceinline__ VoxJob vox_job(const VoxJob* J) {
 // real field crosses HBM, and each wave moves between its transforms and
 // its Newton solves on its own, so the memory-bound and the FP64-bound work
 // of different waves overlap on a CU.  The operations are those of
-// k_z_c2r256, the single form with zf.linear set (a block takes 64
-// consecutive voxels) and k_z_r2c256, so the bits are the same.
+// k_z_c2r256, the single form NR_ROWS256 (a block takes the 4 runs of a z
+// row) and k_z_r2c256, so the bits are the same.  The lanes add up
+// the 12 floating-point sums of their voxels over the row's runs in LDS (in
+// the FP64 form, in the transforms' exchange buffer, idle during the runs),
+// and the wave writes one partial row per z row: one reduction per wave
+// instead of one per run.
 // NR_ZFUSED512 (Nz = 512) is the same with rows of 512 voxels (8 runs):
-// a pencil takes 32 lanes (the 16 x 16 x 2 form of k_z_c2r512 / k_z_r2c512),
+// a pencil takes 32 lanes (the 16 x 16 x 2 form of k_z_c2r512 / k_z_r2c512;
+// the single form with zf.linear = 1; one partial row per run of 64 voxels),
 // so the wave transforms the component pairs (0, 1) and (2, 3) side by side
 // and component 4 alone, and the transforms' LDS lies inside the row's
 // 5 x 512 values (20 KB in FP64: eight waves fill a CU's 160 KB).
-enum NrMode { NR_SINGLE = 0, NR_BATCHED = 1, NR_ZFUSED = 2, NR_ZFUSED512 = 3 };
+// NR_ROWS256 is the single form of the Nz = 256 fused kernel's Newton part:
+// a block takes a z row of 256 voxels (d~ and phi in HBM), with the same
+// LDS row sums and one partial row.
+enum NrMode { NR_SINGLE = 0, NR_BATCHED = 1, NR_ZFUSED = 2, NR_ZFUSED512 = 3, NR_ROWS256 = 4 };
 
 struct ZFusedArgs {
   const void* spec_in;
@@ -607,7 +617,8 @@ struct ZFusedArgs {
   long nc;         // spectrum stride per component
   int nzp;         // spectrum row stride
   double scale;    // 1/N of the inverse transform
-  int linear;      // NR_SINGLE: blocks take 64 consecutive voxels (the fused form's runs)
+  int linear;      // NR_SINGLE: 0, a block takes a 4 x 16 tile; 1, 64 consecutive voxels (a run
+                   // of the Nz = 512 fused form)
 };
 
 template <typename T, int NE, bool UNIFORM_TAU, int MODE>
@@ -713,17 +724,35 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
     __syncthreads();
   }
   const StepConst& C = *as_const_mem(Cp);
-  for (int q = 0; q < (FUSED ? RUNS : 1); q++) {
+  // Rows of 256 voxels (the fused kernel, and the single form with
+  // zf.linear = 4) write one partial row per wave: each lane keeps the 12
+  // floating-point sums of its voxels in LDS across the runs (kRowSums x 64
+  // doubles; the FP64 fused kernel uses its idle exchange buffer), the step
+  // and degenerate counts, the maxima and the wave's steps in registers.
+  constexpr int kRowSums = 12;
+  constexpr bool ZBUF_ACC = MODE == NR_ZFUSED && sizeof(T) == 8;
+  static_assert(!ZBUF_ACC || zbuf_reals<3>() >= kRowSums * 64, "row sums fit the exchange buffer");
+  __shared__ double accbuf[(MODE == NR_ROWS256 || (MODE == NR_ZFUSED && !ZBUF_ACC)) ? kRowSums * 64 : 1];
+  double* accl = ZBUF_ACC ? (double*)(void*)zbuf : accbuf;
+  constexpr bool row_sums = MODE == NR_ZFUSED || MODE == NR_ROWS256;
+  constexpr int nruns = FUSED ? RUNS : (MODE == NR_ROWS256 ? 4 : 1);
+  // (the counts as integers: fewer registers live through the solves)
+  int rsteps = 0, rdegen = 0, rmaxsteps = 0, wsteps = 0;
+  double rmaxerr = 0.0;
+  for (int q = 0; q < nruns; q++) {
     double acc[15];
 #pragma unroll
     for (int w = 0; w < 15; w++) acc[w] = 0.0;
     double mx[2] = {0.0, 0.0};  // max NR error, max NR steps
+    double run_steps = 0.0;
     // one voxel per lane: nothing but the voxel's own state is live
     long v, slot = 0;
     if constexpr (FUSED) {
       slot = q * 64 + (threadIdx.x & 63);
       v = zrow * ZN + slot;
       blk = zrow * RUNS + q;
+    } else if (MODE == NR_ROWS256) {
+      v = (blk * nruns + q) * kNrThreads + threadIdx.x;
     } else if (MODE == NR_SINGLE && zf.linear) {
       v = blk * kNrThreads + threadIdx.x;
     } else {
@@ -825,6 +854,7 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
       acc[12] += (double)steps;
       mx[0] = fmax(mx[0], err);
       mx[1] = fmax(mx[1], (double)steps);
+      run_steps = (double)steps;
       // the first non-finite value of the step: the FFT's output, then what
       // this kernel made of it
       const double fd = sum5(d), fs = sum5(s), fe = sum5(e), fl = sum5(lam);
@@ -836,10 +866,33 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
         report_health(health, HK_NR, f, ls ? ls->iter + 1 : 0, v, val);
       }
     }
-    // lane efficiency of the Newton loop: a wave runs as many steps as its
-    // slowest lane, the wave's maximum of the steps (slot 14 is 0 in every
-    // lane before the reduction and 64 x that maximum after it)
-    wave_partial_row(acc, mx[0], mx[1], partial + blk * kPartialStride);
+    if constexpr (row_sums) {
+      const int lane = threadIdx.x & 63;
+#pragma unroll
+      for (int w = 0; w < kRowSums; w++) accl[w * 64 + lane] = q == 0 ? acc[w] : accl[w * 64 + lane] + acc[w];
+      rsteps += (int)acc[12];
+      rdegen += (int)acc[13];
+      rmaxerr = fmax(rmaxerr, mx[0]);
+      rmaxsteps = max(rmaxsteps, (int)mx[1]);
+      // lane efficiency: a wave runs as many steps as its slowest lane, per run
+      wsteps += __builtin_amdgcn_readfirstlane((int)wave_max_all(run_steps));
+    } else {
+      // lane efficiency of the Newton loop: a wave runs as many steps as its
+      // slowest lane, the wave's maximum of the steps (slot 14 is 0 in every
+      // lane before the reduction and 64 x that maximum after it)
+      wave_partial_row(acc, mx[0], mx[1], partial + blk * kPartialStride);
+    }
+  }
+  if constexpr (row_sums) {
+    const int lane = threadIdx.x & 63;
+    double tot[15];
+#pragma unroll
+    for (int w = 0; w < kRowSums; w++) tot[w] = accl[w * 64 + lane];
+    tot[12] = (double)rsteps;
+    tot[13] = (double)rdegen;
+    tot[14] = 0.0;
+    wave_partial_row(tot, rmaxerr, (double)rmaxsteps, partial + (FUSED ? zrow : (long)blockIdx.x) * kPartialStride,
+                     64.0 * (double)wsteps);
   }
   if constexpr (MODE == NR_ZFUSED512) {  // phi row: forward z transforms from LDS
     // the pairs (0, 1) and (2, 3) first, their transform's LDS in the rows of
@@ -1380,13 +1433,19 @@ hipError_t launch_derive_from_L0(StepConst* C, GreenConst<double>* g64, GreenCon
 
 hipError_t launch_nr(int prec, int nexp_uniform, bool uniform_tau, long n, const void* dtil, void* phi, double* sigma,
                      double* lambda, const double* R, const double* crss, const StepConst* C, double* partial,
-                     const LoopState* ls, HealthRecord* health, bool linear, hipStream_t st) {
-  const int nb = nr_blocks(n);
+                     const LoopState* ls, HealthRecord* health, int row_runs, hipStream_t st) {
+  if (row_runs != 0 && row_runs != 1 && row_runs != 4) return hipErrorInvalidValue;
+  if (row_runs == 4 && n % 256) return hipErrorInvalidValue;
+  const int nb = nr_rows(n, row_runs);
   ZFusedArgs zf{};
-  zf.linear = linear ? 1 : 0;
+  zf.linear = row_runs == 1 ? 1 : 0;
 #define NR_LAUNCH(TT, NN, UU)                                                                                    \
-  hipLaunchKernelGGL((k_nr<TT, NN, UU, NR_SINGLE>), dim3(nb), dim3(kNrThreads), 0, st, n, (const TT*)dtil,      \
-                     (TT*)phi, sigma, lambda, R, crss, C, partial, ls, health, VoxGroup{}, zf)
+  if (row_runs == 4)                                                                                             \
+    hipLaunchKernelGGL((k_nr<TT, NN, UU, NR_ROWS256>), dim3(nb), dim3(kNrThreads), 0, st, n, (const TT*)dtil,   \
+                       (TT*)phi, sigma, lambda, R, crss, C, partial, ls, health, VoxGroup{}, zf);                 \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_nr<TT, NN, UU, NR_SINGLE>), dim3(nb), dim3(kNrThreads), 0, st, n, (const TT*)dtil,    \
+                       (TT*)phi, sigma, lambda, R, crss, C, partial, ls, health, VoxGroup{}, zf)
 #define NR_N(TT, UU)                                 \
   switch (nexp_uniform) {                             \
     case 5: NR_LAUNCH(TT, 5, UU); break;              \
@@ -1443,8 +1502,9 @@ hipError_t launch_zfused(int prec, int nexp_uniform, bool uniform_tau, long n, c
 #undef ZF_LAUNCH
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  // the same partial rows as k_nr's, folded the same way
-  const int nb = nr_blocks(n);
+  // the same partial rows as launch_nr's with row_runs = 4 (Nz = 256: one per
+  // z row) or 1 (Nz = 512: one per run of 64 voxels), folded the same way
+  const int nb = nr_rows(n, g.nz == 256 ? 4 : 1);
   const int nb2 = (nb + 255) / 256;
   hipLaunchKernelGGL(k_fold_partials, dim3(nb2), dim3(256), 0, st, partial, nb, partial + (long)nb * kPartialStride,
                      ls ? &ls->done : nullptr);
@@ -1462,10 +1522,10 @@ hipError_t launch_phi_rebuild(int prec, long n, const void* dtil, void* phi, con
   return hipGetLastError();
 }
 
-hipError_t launch_iter_finalize(long n, const double* partial, const StepConst* C, double tol,
+hipError_t launch_iter_finalize(long n, int row_runs, const double* partial, const StepConst* C, double tol,
                                 int max_iter, LoopState* ls, IterRecord* rec, hipStream_t st) {
-  // launch_nr left ceil(nr_blocks / 256) folded rows after the per-block partials
-  const int nb = nr_blocks(n);
+  // launch_nr / launch_zfused left ceil(rows / 256) folded rows after the per-wave partials
+  const int nb = nr_rows(n, row_runs);
   const int nb2 = (nb + 255) / 256;
   hipLaunchKernelGGL(k_iter_finalize, dim3(1), dim3(256), 0, st, n, nb2, partial + (long)nb * kPartialStride, C,
                      tol, max_iter, ls, rec);
