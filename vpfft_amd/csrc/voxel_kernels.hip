@@ -810,10 +810,8 @@ __global__ void __launch_bounds__(kNrThreads) __attribute__((amdgpu_waves_per_eu
       // rate from the crystal frame, e = Q^T ec
       double s[5], e[5];
       {
-        // the orientation again (one more read of it, not 18 registers live
-        // through the solve): through a pointer the compiler cannot match
-        // with the first read
-
+        // Q(R) formed again from the orientation's 9 values, which stay live
+        // through the solve (Q's 25 would not fit beside the solve's state)
         double Q[25], ec[5], gs[5];
         cf::dev_map(r9, Q);
         cf::apply_constitutive<NE, UNIFORM_TAU && (NE > 0)>(sc, it, C, ec);
